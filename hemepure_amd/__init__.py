@@ -1,0 +1,384 @@
+"""Python plumbing over the hemepure_amd C ABI (include/hemepure_amd.h).
+
+The library is the product: HIP kernels for gfx950 plus the host-side lattice
+builder behind an ``extern "C"`` boundary.  This module only binds it with
+ctypes for tests and benchmarks; it never computes anything itself and raises
+if the native library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libhemepure_amd.so"
+
+# Constants of include/hemepure_amd.h.
+LINK_NONE, LINK_WALL, LINK_INLET, LINK_OUTLET = 0, 1, 2, 3
+KERNEL_LBGK, KERNEL_TRT, KERNEL_LES = 0, 1, 2
+WALL_SBB, WALL_BFL = 0, 1
+IOLET_NASH, IOLET_LADD = 0, 1
+COSINE, PARABOLIC, FILEVEL, WK2, WK3 = 0, 1, 2, 3, 4
+FP64, FP32_SHIFTED = 0, 1
+STEP_PROPERTIES = 1
+Q = 19
+NO_VALUE = np.finfo(np.float64).max
+
+
+class HlbError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"hemepure_amd error {code}: {msg}")
+        self.code = code
+
+
+class GeometryView(C.Structure):
+    _fields_ = [
+        ("blocks", C.c_int32 * 3),
+        ("block_size", C.c_int32),
+        ("n_sites", C.c_int64),
+        ("coords", C.POINTER(C.c_int32)),
+        ("link_type", C.POINTER(C.c_uint8)),
+        ("link_iolet", C.POINTER(C.c_int32)),
+        ("link_dist", C.POINTER(C.c_float)),
+        ("normal_avail", C.POINTER(C.c_uint8)),
+        ("normal", C.POINTER(C.c_float)),
+    ]
+
+
+class IoletDesc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("warmup", C.c_int32),
+        ("position", C.c_double * 3),
+        ("normal", C.c_double * 3),
+        ("radius", C.c_double),
+        ("density_mean", C.c_double),
+        ("density_amp", C.c_double),
+        ("phase", C.c_double),
+        ("period", C.c_double),
+        ("max_speed", C.c_double),
+        ("table", C.POINTER(C.c_double)),
+        ("table_len", C.c_int64),
+        ("area", C.c_double),
+        ("resistance", C.c_double),
+        ("r_c", C.c_double),
+        ("r_p", C.c_double),
+        ("capacitance", C.c_double),
+    ]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("kernel", C.c_int32),
+        ("wall_bc", C.c_int32),
+        ("inlet_bc", C.c_int32),
+        ("outlet_bc", C.c_int32),
+        ("precision", C.c_int32),
+        ("reserved", C.c_int32),
+        ("tau", C.c_double),
+        ("trt_lambda", C.c_double),
+    ]
+
+
+class IoletGeom(C.Structure):
+    _fields_ = [
+        ("is_inlet", C.c_int32),
+        ("id", C.c_int32),
+        ("position", C.c_double * 3),
+        ("normal", C.c_double * 3),
+        ("radius", C.c_double),
+    ]
+
+
+_lib = None
+
+_SIGS = {
+    "hlb_last_error": (C.c_char_p, []),
+    "hlb_create": (C.c_int, [C.POINTER(GeometryView), C.POINTER(Params), C.POINTER(IoletDesc), C.c_int,
+                             C.POINTER(IoletDesc), C.c_int, C.POINTER(C.c_void_p)]),
+    "hlb_destroy": (None, [C.c_void_p]),
+    "hlb_init_equilibrium": (C.c_int, [C.c_void_p, C.c_double, C.c_int64]),
+    "hlb_upload_f": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "hlb_step": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint32]),
+    "hlb_step_timed": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint32, C.POINTER(C.c_double)]),
+    "hlb_synchronize": (C.c_int, [C.c_void_p]),
+    "hlb_download_f": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hlb_get_fields": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hlb_timestep": (C.c_int64, [C.c_void_p]),
+    "hlb_stability": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "hlb_site_count": (C.c_int64, [C.c_void_p]),
+    "hlb_class_counts": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hlb_site_coords": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hlb_neighbour_tables": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hlb_site_masks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hlb_wall_geometry": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hlb_geom_cylinder": (C.c_int, [C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "hlb_geom_ybifurcation": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "hlb_geom_stenosed_pipe": (C.c_int, [C.c_double, C.c_int32, C.c_double, C.c_double,
+                                         C.POINTER(C.c_void_p)]),
+    "hlb_geom_view": (C.c_int, [C.c_void_p, C.POINTER(GeometryView)]),
+    "hlb_geom_iolets": (C.c_int, [C.c_void_p, C.POINTER(IoletGeom), C.c_int]),
+    "hlb_geom_free": (None, [C.c_void_p]),
+}
+
+
+def lib() -> C.CDLL:
+    """Loads the native library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m hemepure_amd.build` "
+                "(there is no fallback implementation)")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        raise HlbError(rc, lib().hlb_last_error().decode())
+    return rc
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+class Geometry:
+    """Per-site geometry records: what GeometryReader yields from a .gmy file."""
+
+    def __init__(self, blocks, coords, link_type, link_iolet, link_dist, normal_avail, normal,
+                 iolets=None, block_size=8, _owner=None):
+        self.blocks = tuple(int(b) for b in blocks)
+        self.block_size = int(block_size)
+        self.coords = np.ascontiguousarray(coords, dtype=np.int32).reshape(-1, 3)
+        n = self.coords.shape[0]
+        self.link_type = np.ascontiguousarray(link_type, dtype=np.uint8).reshape(n, 18)
+        self.link_iolet = np.ascontiguousarray(link_iolet, dtype=np.int32).reshape(n, 18)
+        self.link_dist = np.ascontiguousarray(link_dist, dtype=np.float32).reshape(n, 18)
+        self.normal_avail = np.ascontiguousarray(normal_avail, dtype=np.uint8).reshape(n)
+        self.normal = np.ascontiguousarray(normal, dtype=np.float32).reshape(n, 3)
+        self.iolets = list(iolets or [])
+        self._owner = _owner
+
+    @property
+    def n_sites(self) -> int:
+        return int(self.coords.shape[0])
+
+    def view(self) -> GeometryView:
+        v = GeometryView()
+        v.blocks[:] = self.blocks
+        v.block_size = self.block_size
+        v.n_sites = self.n_sites
+        v.coords = self.coords.ctypes.data_as(C.POINTER(C.c_int32))
+        v.link_type = self.link_type.ctypes.data_as(C.POINTER(C.c_uint8))
+        v.link_iolet = self.link_iolet.ctypes.data_as(C.POINTER(C.c_int32))
+        v.link_dist = self.link_dist.ctypes.data_as(C.POINTER(C.c_float))
+        v.normal_avail = self.normal_avail.ctypes.data_as(C.POINTER(C.c_uint8))
+        v.normal = self.normal.ctypes.data_as(C.POINTER(C.c_float))
+        return v
+
+    @classmethod
+    def _from_native(cls, gptr: C.c_void_p) -> "Geometry":
+        L = lib()
+        v = GeometryView()
+        try:
+            _check(L.hlb_geom_view(gptr, C.byref(v)))
+            n = v.n_sites
+
+            def arr(p, count, dt):
+                if count == 0:
+                    return np.zeros(0, dtype=dt)
+                return np.ctypeslib.as_array(p, shape=(count,)).astype(dt, copy=True)
+
+            nio = _check(L.hlb_geom_iolets(gptr, None, 0))
+            ios = (IoletGeom * max(nio, 1))()
+            L.hlb_geom_iolets(gptr, ios, nio)
+            iolets = [dict(is_inlet=bool(ios[i].is_inlet), id=int(ios[i].id),
+                           position=tuple(ios[i].position), normal=tuple(ios[i].normal),
+                           radius=float(ios[i].radius)) for i in range(nio)]
+            return cls(tuple(v.blocks), arr(v.coords, 3 * n, np.int32), arr(v.link_type, 18 * n, np.uint8),
+                       arr(v.link_iolet, 18 * n, np.int32), arr(v.link_dist, 18 * n, np.float32),
+                       arr(v.normal_avail, n, np.uint8), arr(v.normal, 3 * n, np.float32), iolets,
+                       v.block_size)
+        finally:
+            L.hlb_geom_free(gptr)
+
+    @classmethod
+    def cylinder(cls, radius: float, length: int, box_xy: int) -> "Geometry":
+        g = C.c_void_p()
+        _check(lib().hlb_geom_cylinder(radius, length, box_xy, C.byref(g)))
+        return cls._from_native(g)
+
+    @classmethod
+    def ybifurcation(cls, rp=24.0, lp=200.0, rd=19.0, ld=160.0, angle_deg=30.0, jitter=0.03,
+                     seed=2026) -> "Geometry":
+        g = C.c_void_p()
+        _check(lib().hlb_geom_ybifurcation(rp, lp, rd, ld, angle_deg, jitter, seed, C.byref(g)))
+        return cls._from_native(g)
+
+    @classmethod
+    def stenosed_pipe(cls, radius: float, length: int, area_reduction: float, width: float) -> "Geometry":
+        g = C.c_void_p()
+        _check(lib().hlb_geom_stenosed_pipe(radius, length, area_reduction, width, C.byref(g)))
+        return cls._from_native(g)
+
+    def inlets(self):
+        return sorted((i for i in self.iolets if i["is_inlet"]), key=lambda i: i["id"])
+
+    def outlets(self):
+        return sorted((i for i in self.iolets if not i["is_inlet"]), key=lambda i: i["id"])
+
+
+def iolet_desc(kind: int, position, normal, radius=0.0, density_mean=1.0, density_amp=0.0, phase=0.0,
+               period=1.0, warmup=0, max_speed=0.0, table=None, area=1.0, resistance=1.0, r_c=1.0,
+               r_p=1.0, capacitance=1.0) -> IoletDesc:
+    d = IoletDesc()
+    d.kind = kind
+    d.warmup = warmup
+    d.position[:] = tuple(position)
+    d.normal[:] = tuple(normal)
+    d.radius = radius
+    d.density_mean = density_mean
+    d.density_amp = density_amp
+    d.phase = phase
+    d.period = period
+    d.max_speed = max_speed
+    if table is not None:
+        t = np.ascontiguousarray(table, dtype=np.float64)
+        d._table_keepalive = t
+        d.table = t.ctypes.data_as(C.POINTER(C.c_double))
+        d.table_len = t.size
+    d.area = area
+    d.resistance = resistance
+    d.r_c = r_c
+    d.r_p = r_p
+    d.capacitance = capacitance
+    return d
+
+
+def params(kernel=KERNEL_LBGK, wall_bc=WALL_SBB, inlet_bc=IOLET_NASH, outlet_bc=IOLET_NASH, tau=0.8,
+           trt_lambda=3.0 / 16.0, precision=FP64) -> Params:
+    p = Params()
+    p.kernel, p.wall_bc, p.inlet_bc, p.outlet_bc = kernel, wall_bc, inlet_bc, outlet_bc
+    p.precision = precision
+    p.tau = tau
+    p.trt_lambda = trt_lambda
+    return p
+
+
+def _desc_array(descs):
+    arr = (IoletDesc * max(len(descs), 1))()
+    for i, d in enumerate(descs):
+        arr[i] = d
+    return arr
+
+
+class Lattice:
+    """One handle: LatticeData + LBM + BoundaryValues of the reference, on one GPU."""
+
+    def __init__(self, geom: Geometry, prm: Params, inlets=(), outlets=()):
+        self._geom = geom
+        self._inlets = list(inlets)
+        self._outlets = list(outlets)
+        v = geom.view()
+        h = C.c_void_p()
+        _check(lib().hlb_create(C.byref(v), C.byref(prm), _desc_array(self._inlets), len(self._inlets),
+                                _desc_array(self._outlets), len(self._outlets), C.byref(h)))
+        self._h = h
+        self.n = int(lib().hlb_site_count(h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hlb_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- tables ------------------------------------------------------------
+    def class_counts(self) -> np.ndarray:
+        out = np.zeros(6, dtype=np.int64)
+        _check(lib().hlb_class_counts(self._h, _ptr(out)))
+        return out
+
+    def site_coords(self) -> np.ndarray:
+        out = np.zeros((self.n, 3), dtype=np.int64)
+        _check(lib().hlb_site_coords(self._h, _ptr(out)))
+        return out
+
+    def neighbour_tables(self):
+        push = np.zeros((self.n, Q), dtype=np.int64)
+        pull = np.zeros((18, self.n), dtype=np.uint32)
+        _check(lib().hlb_neighbour_tables(self._h, _ptr(push), _ptr(pull)))
+        return push, pull
+
+    def site_masks(self):
+        w = np.zeros(self.n, dtype=np.uint32)
+        i = np.zeros(self.n, dtype=np.uint32)
+        ident = np.zeros(self.n, dtype=np.int32)
+        t = np.zeros(self.n, dtype=np.int32)
+        _check(lib().hlb_site_masks(self._h, _ptr(w), _ptr(i), _ptr(ident), _ptr(t)))
+        return w, i, ident, t
+
+    def wall_geometry(self):
+        d = np.zeros((self.n, 18), dtype=np.float64)
+        nrm = np.zeros((self.n, 3), dtype=np.float64)
+        _check(lib().hlb_wall_geometry(self._h, _ptr(d), _ptr(nrm)))
+        return d, nrm
+
+    # -- state ----------------------------------------------------------------
+    def init_equilibrium(self, rho0=1.0, t0=1):
+        _check(lib().hlb_init_equilibrium(self._h, rho0, t0))
+
+    def upload_f(self, f: np.ndarray, t0=1):
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(self.n, Q)
+        _check(lib().hlb_upload_f(self._h, _ptr(f), t0))
+
+    def step(self, nsteps: int, properties=False):
+        _check(lib().hlb_step(self._h, nsteps, STEP_PROPERTIES if properties else 0))
+
+    def step_timed(self, nsteps: int, properties=False) -> float:
+        ms = C.c_double()
+        _check(lib().hlb_step_timed(self._h, nsteps, STEP_PROPERTIES if properties else 0, C.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        _check(lib().hlb_synchronize(self._h))
+
+    def download_f(self) -> np.ndarray:
+        f = np.zeros((self.n, Q), dtype=np.float64)
+        _check(lib().hlb_download_f(self._h, _ptr(f)))
+        return f
+
+    def fields(self):
+        rho = np.zeros(self.n, dtype=np.float64)
+        u = np.zeros((self.n, 3), dtype=np.float64)
+        wss = np.zeros(self.n, dtype=np.float64)
+        _check(lib().hlb_get_fields(self._h, _ptr(rho), _ptr(u), _ptr(wss)))
+        return rho, u, wss
+
+    def timestep(self) -> int:
+        return int(lib().hlb_timestep(self._h))
+
+    def stable(self) -> bool:
+        s = C.c_int()
+        _check(lib().hlb_stability(self._h, C.byref(s)))
+        return bool(s.value)

@@ -1,0 +1,676 @@
+// capi.hip -- the extern "C" boundary: handle lifecycle, device buffers,
+// iolet bookkeeping and the step driver.
+//
+// A handle plays the roles of the reference's LatticeData (tables and f
+// arrays), LBM (kernel/streamer selection and the phase sequence) and the two
+// BoundaryValues objects (iolet state), for one GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/hemepure_amd.h"
+#include "d3q19.h"
+#include "errors.h"
+#include "lattice.h"
+#include "lbm_kernels.h"
+
+namespace hlb {
+
+std::string& last_error() {
+  thread_local std::string msg;
+  return msg;
+}
+
+hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, hipStream_t s);
+hipError_t launch_gather_fp64(const StepArgs& a, double* F, hipStream_t s);
+hipError_t launch_fill_fp64(double* F, int64_t N, int64_t S, const double* vals, hipStream_t s);
+hipError_t launch_unstable_fp64(const double* F, int64_t N, int64_t S, int* flag, hipStream_t s);
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;  // src/constants.h:17
+constexpr int64_t kChunk = 1024;  // steps of iolet values staged per upload
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+struct HostIolet {
+  hlb_iolet_desc d;
+  double n[3];    // InOutLet::SetNormal: normalised in double (InOutLet.h:169-172)
+  float nf[3];    // the Vector3D<float> copy Nash uses (NashZerothOrderPressureDelegate.h:57-58)
+  std::vector<double> table;
+  double minSimDensity = 1.0;
+};
+
+// InOutLetCosine::GetDensity (src/lb/iolets/InOutLetCosine.cc:37-54) at the
+// 0-indexed step t0.
+double cosine_density(const HostIolet& io, uint64_t t0) {
+  const double w = 2.0 * kPi / io.d.period;
+  const double target = io.d.density_mean + io.d.density_amp * std::cos(w * t0 + io.d.phase);
+  const uint32_t warm = (uint32_t)io.d.warmup;
+  if (t0 >= warm) return target;
+  const double f = ((double)t0) / ((double)warm);
+  return f * target + (1. - f) * io.minSimDensity;
+}
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t count) {
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+    }
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc(&p, sizeof(T) * count);
+  }
+};
+
+}  // namespace
+}  // namespace hlb
+
+struct hlb_lattice {
+  hlb::HostLattice L;
+  hlb_params params{};
+  std::vector<hlb::HostIolet> iolets;  // inlets then outlets
+  int n_in = 0, n_out = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int64_t S = 0, NB = 0, SB = 0, NI = 0, SI = 0;
+  hlb::DevBuf<double> A[2], F, G, ladd_s, rho, u, wss, io_n, io_dens, io_vmax, feq_vals;
+  hlb::DevBuf<uint32_t> pull, bwall, biolet;
+  hlb::DevBuf<int32_t> bgi;
+  hlb::DevBuf<float> bq, bnrm, io_nf;
+  hlb::DevBuf<int> flag;
+  double* h_stage = nullptr;  // pinned staging for per-step iolet values
+  int cur = 0;                // A[cur] holds the latest post-collision values
+  bool have_post = false;     // false: the next step reads the pre-collision F
+  bool f_valid = false;
+  int64_t t = 1;              // 1-indexed step that runs next
+  double omega = 0, half_omega = 0, half_omega_minus = 0, les_k = 0;
+  std::vector<double> h_ladd_s;  // host copy of the Ladd profile factors (18 planes of SI)
+  bool device_ready = false;
+
+  ~hlb_lattice() {
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (h_stage) (void)hipHostFree(h_stage);
+  }
+};
+
+namespace hlb {
+namespace {
+
+int dev_fail(hipError_t e, const char* what) {
+  return fail(HLB_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HLB_CHECK(expr)                                 \
+  do {                                                  \
+    hipError_t _e = (expr);                             \
+    if (_e != hipSuccess) return dev_fail(_e, #expr);   \
+  } while (0)
+
+StepArgs make_args(hlb_lattice* h, const double* ain, double* aout) {
+  StepArgs a{};
+  a.Ain = ain;
+  a.Aout = aout;
+  a.pull = h->pull.p;
+  a.bwall = h->bwall.p;
+  a.biolet = h->biolet.p;
+  a.bgi = h->bgi.p;
+  a.bq = h->bq.p;
+  a.bnrm = h->bnrm.p;
+  a.G = h->G.p;
+  a.ladd_s = h->ladd_s.p;
+  a.rho = h->rho.p;
+  a.u = h->u.p;
+  a.wss = h->wss.p;
+  a.io_nf = h->io_nf.p;
+  a.io_n = h->io_n.p;
+  a.io_dens = h->io_dens.p;
+  a.io_vmax = h->io_vmax.p;
+  a.N = h->L.N;
+  a.S = h->S;
+  a.off1 = h->L.off[1];
+  a.off2 = h->L.off[2];
+  a.SB = h->SB;
+  a.SI = h->SI;
+  a.n_in = h->n_in;
+  a.wall_bc = h->params.wall_bc;
+  a.inlet_bc = h->params.inlet_bc;
+  a.outlet_bc = h->params.outlet_bc;
+  a.tau = h->params.tau;
+  a.omega = h->omega;
+  a.half_omega = h->half_omega;
+  a.half_omega_minus = h->half_omega_minus;
+  a.les_k = h->les_k;
+  return a;
+}
+
+// Ghost density (Nash) and peak speed (Ladd) of every iolet at 1-indexed step t.
+int iolet_values(const hlb_lattice* h, int64_t t, double* dens, double* vmax) {
+  const int nio = (int)h->iolets.size();
+  for (int g = 0; g < nio; ++g) {
+    const HostIolet& io = h->iolets[g];
+    dens[g] = 1.0;
+    vmax[g] = 0.0;
+    switch (io.d.kind) {
+      case HLB_IOLET_COSINE: dens[g] = cosine_density(io, (uint64_t)(t - 1)); break;
+      case HLB_IOLET_PARABOLIC: {
+        // InOutLetParabolicVelocity.cc:47-52
+        double max = io.d.max_speed;
+        const uint32_t warm = (uint32_t)io.d.warmup;
+        if ((uint64_t)t < warm) max *= (uint64_t)t / double(warm);
+        vmax[g] = max;
+        break;
+      }
+      case HLB_IOLET_FILEVEL:
+        if (t >= (int64_t)io.table.size())
+          return fail(HLB_ERANGE, "velocity table of iolet " + std::to_string(g) + " ends before step " +
+                                      std::to_string(t));
+        vmax[g] = io.table[t];
+        break;
+      default: break;
+    }
+  }
+  return HLB_OK;
+}
+
+// Ladd profile factors s = 1 - r^2/a^2 at each velocity link's half-way
+// point (LaddIoletDelegate.h:66-69, InOutLetParabolicVelocity.cc:30-45).  The
+// reference stops with std::exit(16) when a half-way point lies outside the
+// iolet radius; here hlb_create fails with HLB_ERANGE.
+int ladd_factors(hlb_lattice* h) {
+  const HostLattice& L = h->L;
+  h->NI = L.N - L.off[2];
+  h->SI = round_up(std::max<int64_t>(h->NI, 1), 64);
+  const int64_t SI = h->SI;
+  std::vector<double>& s = h->h_ladd_s;
+  s.assign((size_t)18 * SI, 0.0);
+  for (int64_t i = 0; i < h->NI; ++i) {
+    const int64_t k = L.off[2] + i;
+    const int c = L.collision_class(k);
+    const bool inletSide = (c == 2 || c == 4);
+    const int bc = inletSide ? h->params.inlet_bc : h->params.outlet_bc;
+    if (bc != HLB_IOLET_LADD) continue;
+    const int g = inletSide ? L.ioletId[k] : h->n_in + L.ioletId[k];
+    const HostIolet& io = h->iolets[g];
+    for (int d = 1; d < Q; ++d) {
+      if (!((L.iolet[k] >> (d - 1)) & 1u)) continue;
+      const double hx = (double)L.coords[3 * k] + 0.5 * CX[d];
+      const double hy = (double)L.coords[3 * k + 1] + 0.5 * CY[d];
+      const double hz = (double)L.coords[3 * k + 2] + 0.5 * CZ[d];
+      const double dx = hx - io.d.position[0], dy = hy - io.d.position[1], dz = hz - io.d.position[2];
+      const double z = dx * io.n[0] + dy * io.n[1] + dz * io.n[2];
+      const double rSq = (dx * dx + dy * dy + dz * dz) - z * z;
+      const double r2a2 = rSq / (io.d.radius * io.d.radius);
+      if (r2a2 > 1.0) {
+        char buf[200];
+        std::snprintf(buf, sizeof buf, "An IOLET site with r = %lf lies outside the IOLET radius %lf.",
+                      std::sqrt(rSq), io.d.radius);
+        return fail(HLB_ERANGE, buf);
+      }
+      s[(size_t)(d - 1) * SI + i] = 1. - r2a2;
+    }
+  }
+  return HLB_OK;
+}
+
+int upload_lattice(hlb_lattice* h) {
+  const HostLattice& L = h->L;
+  const int64_t N = L.N;
+  h->S = round_up(std::max<int64_t>(N, 1), 64);
+  h->NB = N - L.off[1];
+  h->SB = round_up(std::max<int64_t>(h->NB, 1), 64);
+  const int64_t S = h->S, SB = h->SB, SI = h->SI;
+  HLB_CHECK(h->A[0].alloc((size_t)Q * S));
+  HLB_CHECK(h->A[1].alloc((size_t)Q * S));
+  HLB_CHECK(h->pull.alloc((size_t)18 * S));
+  HLB_CHECK(h->bwall.alloc((size_t)SB));
+  HLB_CHECK(h->biolet.alloc((size_t)SB));
+  HLB_CHECK(h->bgi.alloc((size_t)SB));
+  HLB_CHECK(h->bq.alloc((size_t)18 * SB));
+  HLB_CHECK(h->bnrm.alloc((size_t)3 * SB));
+  HLB_CHECK(h->G.alloc((size_t)Q * SI));
+  HLB_CHECK(h->ladd_s.alloc((size_t)18 * SI));
+  HLB_CHECK(h->rho.alloc((size_t)S));
+  HLB_CHECK(h->u.alloc((size_t)3 * S));
+  HLB_CHECK(h->wss.alloc((size_t)S));
+  HLB_CHECK(h->flag.alloc(1));
+  HLB_CHECK(h->feq_vals.alloc(Q));
+  HLB_CHECK(hipMemsetAsync(h->A[0].p, 0, sizeof(double) * Q * S, h->stream));
+  HLB_CHECK(hipMemsetAsync(h->A[1].p, 0, sizeof(double) * Q * S, h->stream));
+  HLB_CHECK(hipMemsetAsync(h->G.p, 0, sizeof(double) * Q * SI, h->stream));
+
+  // Pull table, padded planes.
+  {
+    std::vector<uint32_t> tmp((size_t)18 * S, 0);
+    for (int j = 0; j < 18; ++j)
+      std::memcpy(&tmp[(size_t)j * S], &L.pull[(size_t)j * N], sizeof(uint32_t) * N);
+    HLB_CHECK(hipMemcpy(h->pull.p, tmp.data(), sizeof(uint32_t) * tmp.size(), hipMemcpyHostToDevice));
+  }
+  // Boundary arrays.
+  {
+    const int64_t NB = h->NB, o1 = L.off[1];
+    std::vector<uint32_t> w(SB, 0), io(SB, 0);
+    std::vector<int32_t> gi(SB, -1);
+    std::vector<float> q((size_t)18 * SB, 0.0f), nr((size_t)3 * SB, 0.0f);
+    for (int64_t b = 0; b < NB; ++b) {
+      const int64_t k = o1 + b;
+      w[b] = L.wall[k];
+      io[b] = L.iolet[k];
+      if (L.iolet[k] != 0) {
+        const int c = L.collision_class(k);
+        const bool inletSide = (c == 2 || c == 4);
+        gi[b] = inletSide ? L.ioletId[k] : h->n_in + L.ioletId[k];
+      }
+      for (int d = 0; d < 18; ++d) q[(size_t)d * SB + b] = L.q[k * 18 + d];
+      for (int a = 0; a < 3; ++a) nr[(size_t)a * SB + b] = L.nrm[k * 3 + a];
+    }
+    HLB_CHECK(hipMemcpy(h->bwall.p, w.data(), sizeof(uint32_t) * SB, hipMemcpyHostToDevice));
+    HLB_CHECK(hipMemcpy(h->biolet.p, io.data(), sizeof(uint32_t) * SB, hipMemcpyHostToDevice));
+    HLB_CHECK(hipMemcpy(h->bgi.p, gi.data(), sizeof(int32_t) * SB, hipMemcpyHostToDevice));
+    HLB_CHECK(hipMemcpy(h->bq.p, q.data(), sizeof(float) * q.size(), hipMemcpyHostToDevice));
+    HLB_CHECK(hipMemcpy(h->bnrm.p, nr.data(), sizeof(float) * nr.size(), hipMemcpyHostToDevice));
+  }
+  HLB_CHECK(hipMemcpy(h->ladd_s.p, h->h_ladd_s.data(), sizeof(double) * h->h_ladd_s.size(),
+                      hipMemcpyHostToDevice));
+  // Per-iolet constants and the staging for per-step values.
+  {
+    const int nio = std::max<int>(1, (int)h->iolets.size());
+    std::vector<float> nf((size_t)3 * nio, 0.0f);
+    std::vector<double> nd((size_t)3 * nio, 0.0);
+    for (size_t g = 0; g < h->iolets.size(); ++g)
+      for (int a = 0; a < 3; ++a) {
+        nf[3 * g + a] = h->iolets[g].nf[a];
+        nd[3 * g + a] = h->iolets[g].n[a];
+      }
+    HLB_CHECK(h->io_nf.alloc(nf.size()));
+    HLB_CHECK(h->io_n.alloc(nd.size()));
+    HLB_CHECK(h->io_dens.alloc((size_t)kChunk * nio));
+    HLB_CHECK(h->io_vmax.alloc((size_t)kChunk * nio));
+    HLB_CHECK(hipMemcpy(h->io_nf.p, nf.data(), sizeof(float) * nf.size(), hipMemcpyHostToDevice));
+    HLB_CHECK(hipMemcpy(h->io_n.p, nd.data(), sizeof(double) * nd.size(), hipMemcpyHostToDevice));
+    HLB_CHECK(hipHostMalloc((void**)&h->h_stage, sizeof(double) * 2 * kChunk * nio, hipHostMallocDefault));
+  }
+  HLB_CHECK(hipStreamSynchronize(h->stream));
+  return HLB_OK;
+}
+
+// Device state is created on first use so the host tables can be built and
+// inspected on a machine without a GPU.
+int ensure_device(hlb_lattice* h) {
+  if (h->device_ready) return HLB_OK;
+  HLB_CHECK(hipGetDevice(&h->device));
+  HLB_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  HLB_CHECK(hipEventCreate(&h->ev0));
+  HLB_CHECK(hipEventCreate(&h->ev1));
+  const int rc = upload_lattice(h);
+  if (rc != HLB_OK) return rc;
+  h->device_ready = true;
+  return HLB_OK;
+}
+
+int validate_params(const hlb_params& p, const hlb_iolet_desc* in, int nin, const hlb_iolet_desc* out, int nout) {
+  if (p.kernel < HLB_KERNEL_LBGK || p.kernel > HLB_KERNEL_LES) return fail(HLB_EINVAL, "hlb_create: unknown kernel");
+  if (p.wall_bc != HLB_WALL_SBB && p.wall_bc != HLB_WALL_BFL) return fail(HLB_EINVAL, "hlb_create: unknown wall BC");
+  for (int bc : {p.inlet_bc, p.outlet_bc})
+    if (bc != HLB_IOLET_NASH && bc != HLB_IOLET_LADD) return fail(HLB_EINVAL, "hlb_create: unknown iolet BC");
+  if (p.precision != HLB_FP64) return fail(HLB_EINVAL, "hlb_create: only HLB_FP64 storage is available");
+  if (!(p.tau > 0.5)) return fail(HLB_EINVAL, "hlb_create: tau must exceed 1/2");
+  for (int side = 0; side < 2; ++side) {
+    const hlb_iolet_desc* v = side ? out : in;
+    const int n = side ? nout : nin;
+    const int bc = side ? p.outlet_bc : p.inlet_bc;
+    if (n > 0 && !v) return fail(HLB_EINVAL, "hlb_create: null iolet array");
+    for (int i = 0; i < n; ++i) {
+      const int k = v[i].kind;
+      if (k < HLB_IOLET_COSINE || k > HLB_IOLET_WK3) return fail(HLB_EINVAL, "hlb_create: unknown iolet kind");
+      if (k == HLB_IOLET_WK2 || k == HLB_IOLET_WK3)
+        return fail(HLB_EINVAL, "hlb_create: Windkessel iolets are not available in this build");
+      const bool vel = (k == HLB_IOLET_PARABOLIC || k == HLB_IOLET_FILEVEL);
+      if (bc == HLB_IOLET_LADD && !vel)
+        return fail(HLB_EINVAL, "hlb_create: the Ladd boundary needs a velocity iolet (LaddIoletDelegate.h:51-53)");
+      if (vel && !(v[i].radius > 0.0)) return fail(HLB_EINVAL, "hlb_create: velocity iolet without radius");
+      if (k == HLB_IOLET_FILEVEL && (!v[i].table || v[i].table_len <= 0))
+        return fail(HLB_EINVAL, "hlb_create: file velocity iolet without a table");
+      if (k == HLB_IOLET_COSINE && !(v[i].period != 0.0))
+        return fail(HLB_EINVAL, "hlb_create: cosine iolet with zero period");
+    }
+  }
+  return HLB_OK;
+}
+
+int run_steps(hlb_lattice* h, int64_t nsteps, uint32_t flags) {
+  if (nsteps < 0) return fail(HLB_EINVAL, "hlb_step: negative step count");
+  if (!h->f_valid) return fail(HLB_EINVAL, "hlb_step: no initial condition (call hlb_init_equilibrium or hlb_upload_f)");
+  const int nio = std::max<int>(1, (int)h->iolets.size());
+  int64_t done = 0;
+  while (done < nsteps) {
+    const int64_t chunk = std::min<int64_t>(kChunk, nsteps - done);
+    // Stage this chunk's iolet values; the copy waits for the previous
+    // chunk's launches that read the same device buffer (stream order).
+    HLB_CHECK(hipStreamSynchronize(h->stream));
+    double* dens = h->h_stage;
+    double* vmax = h->h_stage + (size_t)kChunk * nio;
+    for (int64_t s = 0; s < chunk; ++s) {
+      const int rc = iolet_values(h, h->t + done + s, dens + s * nio, vmax + s * nio);
+      if (rc != HLB_OK) return rc;
+    }
+    HLB_CHECK(hipMemcpyAsync(h->io_dens.p, dens, sizeof(double) * chunk * nio, hipMemcpyHostToDevice, h->stream));
+    HLB_CHECK(hipMemcpyAsync(h->io_vmax.p, vmax, sizeof(double) * chunk * nio, hipMemcpyHostToDevice, h->stream));
+    for (int64_t s = 0; s < chunk; ++s) {
+      const bool first = !h->have_post;
+      const bool props = (flags & HLB_STEP_PROPERTIES) && (done + s == nsteps - 1);
+      const double* ain = first ? h->F.p : h->A[h->cur].p;
+      double* aout = first ? h->A[h->cur].p : h->A[h->cur ^ 1].p;
+      StepArgs a = make_args(h, ain, aout);
+      a.io_dens = h->io_dens.p + s * nio;
+      a.io_vmax = h->io_vmax.p + s * nio;
+      HLB_CHECK(launch_step_fp64(a, h->params.kernel, first, props, h->stream));
+      if (!first) h->cur ^= 1;
+      h->have_post = true;
+    }
+    done += chunk;
+  }
+  h->t += nsteps;
+  return HLB_OK;
+}
+
+// Pre-collision f entering step t into F (SoA).
+int materialise_f(hlb_lattice* h) {
+  if (!h->have_post) return HLB_OK;  // F already holds it
+  if (!h->F.p) HLB_CHECK(h->F.alloc((size_t)Q * h->S));
+  StepArgs a = make_args(h, h->A[h->cur].p, nullptr);
+  HLB_CHECK(launch_gather_fp64(a, h->F.p, h->stream));
+  return HLB_OK;
+}
+
+}  // namespace
+}  // namespace hlb
+
+using namespace hlb;
+
+extern "C" const char* hlb_last_error(void) { return last_error().c_str(); }
+
+extern "C" int hlb_create(const hlb_geometry* geom, const hlb_params* params, const hlb_iolet_desc* inlets,
+                          int n_inlets, const hlb_iolet_desc* outlets, int n_outlets, hlb_handle* out) {
+  try {
+    if (!geom || !params || !out) return fail(HLB_EINVAL, "hlb_create: null argument");
+    if (n_inlets < 0 || n_outlets < 0) return fail(HLB_EINVAL, "hlb_create: negative iolet count");
+    int rc = validate_params(*params, inlets, n_inlets, outlets, n_outlets);
+    if (rc != HLB_OK) return rc;
+    auto h = std::make_unique<hlb_lattice>();
+    h->params = *params;
+    h->n_in = n_inlets;
+    h->n_out = n_outlets;
+    rc = build_lattice(*geom, h->L);
+    if (rc != HLB_OK) return rc;
+    // Iolet ids on sites must name a configured iolet of the site's side.
+    for (int64_t k = h->L.off[2]; k < h->L.N; ++k) {
+      const int c = h->L.collision_class(k);
+      const int lim = (c == 2 || c == 4) ? n_inlets : n_outlets;
+      if (h->L.ioletId[k] < 0 || h->L.ioletId[k] >= lim)
+        return fail(HLB_EINVAL, "hlb_create: a site names iolet " + std::to_string(h->L.ioletId[k]) +
+                                    " but only " + std::to_string(lim) + " are configured on its side");
+    }
+    for (int side = 0; side < 2; ++side) {
+      const hlb_iolet_desc* v = side ? outlets : inlets;
+      const int n = side ? n_outlets : n_inlets;
+      for (int i = 0; i < n; ++i) {
+        HostIolet io;
+        io.d = v[i];
+        double nx = v[i].normal[0], ny = v[i].normal[1], nz = v[i].normal[2];
+        if (!(nx == 0.0 && ny == 0.0 && nz == 0.0)) {
+          const double mag = std::sqrt(nx * nx + ny * ny + nz * nz);
+          nx /= mag;
+          ny /= mag;
+          nz /= mag;
+        }
+        io.n[0] = nx;
+        io.n[1] = ny;
+        io.n[2] = nz;
+        io.nf[0] = (float)nx;
+        io.nf[1] = (float)ny;
+        io.nf[2] = (float)nz;
+        if (v[i].kind == HLB_IOLET_FILEVEL) io.table.assign(v[i].table, v[i].table + v[i].table_len);
+        io.d.table = nullptr;
+        h->iolets.push_back(std::move(io));
+      }
+    }
+    // LBM::PrepareBoundaryObjects (src/lb/lb.hpp:119-143): every iolet learns
+    // the minimum density any iolet can impose.
+    double minDensity = DBL_MAX;
+    for (const HostIolet& io : h->iolets)
+      minDensity = std::min(minDensity, io.d.kind == HLB_IOLET_COSINE ? io.d.density_mean - io.d.density_amp : 1.0);
+    for (HostIolet& io : h->iolets) io.minSimDensity = minDensity;
+    // LbmParameters (LbmParameters.h:29-45) and the TRT / LES constants.
+    const double tau = params->tau;
+    h->omega = -1.0 / tau;
+    h->half_omega = 0.5 * h->omega;
+    const double tau_minus = 0.5 + params->trt_lambda / (tau - 0.5);
+    h->half_omega_minus = 0.5 * (-1.0 / tau_minus);
+    {
+      const double rho1 = 1.0, C_smag = 0.1;
+      double k = 18.0 * 1.4142135623730950488016887242097;
+      k = k * rho1;
+      k = k * C_smag;
+      k = k * C_smag;
+      h->les_k = k;
+    }
+    rc = ladd_factors(h.get());
+    if (rc != HLB_OK) return rc;
+    *out = h.release();
+    return HLB_OK;
+  } catch (const std::bad_alloc&) {
+    return fail(HLB_EINVAL, "hlb_create: out of host memory");
+  }
+}
+
+extern "C" void hlb_destroy(hlb_handle h) { delete h; }
+
+extern "C" int hlb_init_equilibrium(hlb_handle h, double rho0, int64_t t0) {
+  if (!h) return fail(HLB_EINVAL, "hlb_init_equilibrium: null handle");
+  if (t0 < 1) return fail(HLB_EINVAL, "hlb_init_equilibrium: t0 must be >= 1");
+  {
+    const int rc = ensure_device(h);
+    if (rc != HLB_OK) return rc;
+  }
+  double feq[Q];
+  const FeqPre p = feq_pre(rho0, 0.0, 0.0, 0.0);
+  feq_all(p, 0.0, 0.0, 0.0, feq);
+  if (!h->F.p) HLB_CHECK(h->F.alloc((size_t)Q * h->S));
+  HLB_CHECK(hipMemcpyAsync(h->feq_vals.p, feq, sizeof feq, hipMemcpyHostToDevice, h->stream));
+  HLB_CHECK(launch_fill_fp64(h->F.p, h->L.N, h->S, h->feq_vals.p, h->stream));
+  HLB_CHECK(hipStreamSynchronize(h->stream));
+  h->have_post = false;
+  h->f_valid = true;
+  h->t = t0;
+  return HLB_OK;
+}
+
+extern "C" int hlb_upload_f(hlb_handle h, const double* f, int64_t t0) {
+  if (!h || (!f && h->L.N > 0)) return fail(HLB_EINVAL, "hlb_upload_f: null argument");
+  if (t0 < 1) return fail(HLB_EINVAL, "hlb_upload_f: t0 must be >= 1");
+  {
+    const int rc = ensure_device(h);
+    if (rc != HLB_OK) return rc;
+  }
+  const int64_t N = h->L.N, S = h->S;
+  std::vector<double> soa((size_t)Q * S, 0.0);
+  for (int64_t k = 0; k < N; ++k)
+    for (int j = 0; j < Q; ++j) soa[(size_t)j * S + k] = f[k * Q + j];
+  if (!h->F.p) HLB_CHECK(h->F.alloc((size_t)Q * S));
+  HLB_CHECK(hipMemcpy(h->F.p, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
+  h->have_post = false;
+  h->f_valid = true;
+  h->t = t0;
+  return HLB_OK;
+}
+
+extern "C" int hlb_step(hlb_handle h, int64_t nsteps, uint32_t flags) {
+  if (!h) return fail(HLB_EINVAL, "hlb_step: null handle");
+  return run_steps(h, nsteps, flags);
+}
+
+extern "C" int hlb_step_timed(hlb_handle h, int64_t nsteps, uint32_t flags, double* ms) {
+  if (!h || !ms) return fail(HLB_EINVAL, "hlb_step_timed: null argument");
+  if (nsteps > kChunk) return fail(HLB_EINVAL, "hlb_step_timed: at most 1024 steps per timed call");
+  HLB_CHECK(hipStreamSynchronize(h->stream));
+  // Stage iolet values before the timed region.
+  const int nio = std::max<int>(1, (int)h->iolets.size());
+  double* dens = h->h_stage;
+  double* vmax = h->h_stage + (size_t)kChunk * nio;
+  for (int64_t s = 0; s < nsteps; ++s) {
+    const int rc = iolet_values(h, h->t + s, dens + s * nio, vmax + s * nio);
+    if (rc != HLB_OK) return rc;
+  }
+  HLB_CHECK(hipMemcpyAsync(h->io_dens.p, dens, sizeof(double) * nsteps * nio, hipMemcpyHostToDevice, h->stream));
+  HLB_CHECK(hipMemcpyAsync(h->io_vmax.p, vmax, sizeof(double) * nsteps * nio, hipMemcpyHostToDevice, h->stream));
+  if (!h->f_valid) return fail(HLB_EINVAL, "hlb_step_timed: no initial condition");
+  HLB_CHECK(hipEventRecord(h->ev0, h->stream));
+  for (int64_t s = 0; s < nsteps; ++s) {
+    const bool first = !h->have_post;
+    const bool props = (flags & HLB_STEP_PROPERTIES) && (s == nsteps - 1);
+    const double* ain = first ? h->F.p : h->A[h->cur].p;
+    double* aout = first ? h->A[h->cur].p : h->A[h->cur ^ 1].p;
+    StepArgs a = make_args(h, ain, aout);
+    a.io_dens = h->io_dens.p + s * nio;
+    a.io_vmax = h->io_vmax.p + s * nio;
+    HLB_CHECK(launch_step_fp64(a, h->params.kernel, first, props, h->stream));
+    if (!first) h->cur ^= 1;
+    h->have_post = true;
+  }
+  HLB_CHECK(hipEventRecord(h->ev1, h->stream));
+  HLB_CHECK(hipEventSynchronize(h->ev1));
+  float elapsed = 0.0f;
+  HLB_CHECK(hipEventElapsedTime(&elapsed, h->ev0, h->ev1));
+  *ms = elapsed;
+  h->t += nsteps;
+  return HLB_OK;
+}
+
+extern "C" int hlb_synchronize(hlb_handle h) {
+  if (!h) return fail(HLB_EINVAL, "hlb_synchronize: null handle");
+  if (!h->device_ready) return HLB_OK;
+  HLB_CHECK(hipStreamSynchronize(h->stream));
+  return HLB_OK;
+}
+
+extern "C" int hlb_download_f(hlb_handle h, double* f) {
+  if (!h || (!f && h->L.N > 0)) return fail(HLB_EINVAL, "hlb_download_f: null argument");
+  if (!h->f_valid) return fail(HLB_EINVAL, "hlb_download_f: no state");
+  int rc = materialise_f(h);
+  if (rc != HLB_OK) return rc;
+  const int64_t N = h->L.N, S = h->S;
+  std::vector<double> soa((size_t)Q * S);
+  HLB_CHECK(hipMemcpyAsync(soa.data(), h->F.p, sizeof(double) * soa.size(), hipMemcpyDeviceToHost, h->stream));
+  HLB_CHECK(hipStreamSynchronize(h->stream));
+  for (int64_t k = 0; k < N; ++k)
+    for (int j = 0; j < Q; ++j) f[k * Q + j] = soa[(size_t)j * S + k];
+  return HLB_OK;
+}
+
+extern "C" int hlb_get_fields(hlb_handle h, double* rho, double* u, double* wss) {
+  if (!h) return fail(HLB_EINVAL, "hlb_get_fields: null handle");
+  if (!h->device_ready) return fail(HLB_EINVAL, "hlb_get_fields: no state");
+  const int64_t N = h->L.N, S = h->S;
+  HLB_CHECK(hipStreamSynchronize(h->stream));
+  if (rho) HLB_CHECK(hipMemcpy(rho, h->rho.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+  if (wss) HLB_CHECK(hipMemcpy(wss, h->wss.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+  if (u) {
+    std::vector<double> tmp((size_t)3 * S);
+    HLB_CHECK(hipMemcpy(tmp.data(), h->u.p, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < N; ++k)
+      for (int a = 0; a < 3; ++a) u[3 * k + a] = tmp[(size_t)a * S + k];
+  }
+  return HLB_OK;
+}
+
+extern "C" int64_t hlb_timestep(hlb_handle h) { return h ? h->t : -1; }
+
+extern "C" int hlb_stability(hlb_handle h, int* stable) {
+  if (!h || !stable) return fail(HLB_EINVAL, "hlb_stability: null argument");
+  if (!h->f_valid) return fail(HLB_EINVAL, "hlb_stability: no state");
+  int rc = materialise_f(h);
+  if (rc != HLB_OK) return rc;
+  HLB_CHECK(hipMemsetAsync(h->flag.p, 0, sizeof(int), h->stream));
+  HLB_CHECK(launch_unstable_fp64(h->F.p, h->L.N, h->S, h->flag.p, h->stream));
+  int flag = 0;
+  HLB_CHECK(hipMemcpyAsync(&flag, h->flag.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HLB_CHECK(hipStreamSynchronize(h->stream));
+  *stable = flag ? 0 : 1;
+  return HLB_OK;
+}
+
+extern "C" int64_t hlb_site_count(hlb_handle h) { return h ? h->L.N : -1; }
+
+extern "C" int hlb_class_counts(hlb_handle h, int64_t out[6]) {
+  if (!h || !out) return fail(HLB_EINVAL, "hlb_class_counts: null argument");
+  for (int c = 0; c < 6; ++c) out[c] = h->L.off[c + 1] - h->L.off[c];
+  return HLB_OK;
+}
+
+extern "C" int hlb_site_coords(hlb_handle h, int64_t* out) {
+  if (!h || !out) return fail(HLB_EINVAL, "hlb_site_coords: null argument");
+  for (size_t i = 0; i < h->L.coords.size(); ++i) out[i] = h->L.coords[i];
+  return HLB_OK;
+}
+
+extern "C" int hlb_neighbour_tables(hlb_handle h, int64_t* push, uint32_t* pull) {
+  if (!h) return fail(HLB_EINVAL, "hlb_neighbour_tables: null handle");
+  if (push) push_table(h->L, push);
+  if (pull && !h->device_ready) {
+    std::memcpy(pull, h->L.pull.data(), sizeof(uint32_t) * h->L.pull.size());
+  } else if (pull) {
+    const int64_t N = h->L.N, S = h->S;
+    std::vector<uint32_t> tmp((size_t)18 * S);
+    HLB_CHECK(hipMemcpy(tmp.data(), h->pull.p, sizeof(uint32_t) * tmp.size(), hipMemcpyDeviceToHost));
+    for (int j = 0; j < 18; ++j) std::memcpy(pull + (size_t)j * N, &tmp[(size_t)j * S], sizeof(uint32_t) * N);
+  }
+  return HLB_OK;
+}
+
+extern "C" int hlb_site_masks(hlb_handle h, uint32_t* wall, uint32_t* iolet, int32_t* iolet_id,
+                              int32_t* site_type) {
+  if (!h) return fail(HLB_EINVAL, "hlb_site_masks: null handle");
+  const int64_t N = h->L.N;
+  for (int64_t k = 0; k < N; ++k) {
+    if (wall) wall[k] = h->L.wall[k];
+    if (iolet) iolet[k] = h->L.iolet[k];
+    if (iolet_id) iolet_id[k] = h->L.ioletId[k];
+    if (site_type) site_type[k] = h->L.siteType[k];
+  }
+  return HLB_OK;
+}
+
+extern "C" int hlb_wall_geometry(hlb_handle h, double* dist, double* normal) {
+  if (!h) return fail(HLB_EINVAL, "hlb_wall_geometry: null handle");
+  if (dist)
+    for (size_t i = 0; i < h->L.q.size(); ++i) dist[i] = (double)h->L.q[i];
+  if (normal)
+    for (size_t i = 0; i < h->L.nrm.size(); ++i) normal[i] = (double)h->L.nrm[i];
+  return HLB_OK;
+}

@@ -1,0 +1,201 @@
+// lattice.cpp -- LatticeData construction for one GPU (one rank: every site is
+// mid-domain, there are no halo slots).
+//
+// Classification follows SiteData::SiteData (src/geometry/SiteDataBare.cc:24-76)
+// and the collision-class switch of LatticeData::ProcessReadSites
+// (src/geometry/LatticeData.cc:220-240); the contiguous order is class-major,
+// then block order, then site-in-block order, both z-fastest
+// (LatticeData.h:375-436, src/geometry/VolumeTraverser.cc:56-70).
+//
+// The pull table inverts the reference's push table on plain links: for site k
+// and direction j, the value arriving along j comes from the fluid site at
+// x_k - c_j exactly when k's link INV[j] is a plain link.  Otherwise a
+// boundary delegate of site k itself writes it (SimpleBounceBack, BFL, Nash,
+// Ladd all write fNew[k*19 + INV[d]] for their link d).  Each (site, j) slot
+// must have exactly one writer in the reference's push step; geometries where
+// a link's type disagrees with its neighbour's fluidity give zero or two
+// writers and are rejected.
+#include "lattice.h"
+
+#include <cmath>
+#include <limits>
+#include <string>
+
+#include "d3q19.h"
+#include "errors.h"
+
+namespace hlb {
+
+int64_t HostLattice::site_at(int64_t x, int64_t y, int64_t z) const {
+  const int64_t SX = (int64_t)blocks[0] * bs, SY = (int64_t)blocks[1] * bs, SZ = (int64_t)blocks[2] * bs;
+  if (x < 0 || y < 0 || z < 0 || x >= SX || y >= SY || z >= SZ) return -1;
+  const int64_t b = ((x / bs) * blocks[1] + (y / bs)) * blocks[2] + (z / bs);
+  const int64_t slot = blockSlot[b];
+  if (slot < 0) return -1;
+  const int64_t s = (((x % bs) * bs) + (y % bs)) * bs + (z % bs);
+  return slotSites[slot * (int64_t)bs * bs * bs + s];
+}
+
+int HostLattice::collision_class(int64_t k) const {
+  for (int c = 0; c < 6; ++c)
+    if (k < off[c + 1]) return c;
+  return 5;
+}
+
+namespace {
+
+// Vector3D<float>::GetNormalised (src/util/Vector3D.h:237-265), float math.
+void normalise_float(float v[3]) {
+  if (v[0] == 0.0f && v[1] == 0.0f && v[2] == 0.0f) return;
+  const float mag = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  v[0] /= mag;
+  v[1] /= mag;
+  v[2] /= mag;
+}
+
+}  // namespace
+
+int build_lattice(const hlb_geometry& g, HostLattice& L) {
+  if (g.block_size <= 0 || g.blocks[0] <= 0 || g.blocks[1] <= 0 || g.blocks[2] <= 0 || g.n_sites < 0)
+    return fail(HLB_EINVAL, "hlb_create: bad block grid");
+  if (g.n_sites > 0 && (!g.coords || !g.link_type || !g.link_iolet || !g.link_dist || !g.normal_avail ||
+                        !g.normal))
+    return fail(HLB_EINVAL, "hlb_create: null geometry array");
+  if (g.n_sites >= (int64_t)std::numeric_limits<int32_t>::max())
+    return fail(HLB_EINVAL, "hlb_create: more than 2^31-1 sites per handle");
+  for (int a = 0; a < 3; ++a) L.blocks[a] = g.blocks[a];
+  L.bs = g.block_size;
+  const int64_t bs = g.block_size, spb = bs * bs * bs;
+  const int64_t BX = g.blocks[0], BY = g.blocks[1], BZ = g.blocks[2];
+  const int64_t SX = BX * bs, SY = BY * bs, SZ = BZ * bs;
+  const int64_t n = g.n_sites;
+
+  // Site map over the input records.
+  L.blockSlot.assign(BX * BY * BZ, -1);
+  std::vector<int64_t> slotInput;
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t x = g.coords[3 * i], y = g.coords[3 * i + 1], z = g.coords[3 * i + 2];
+    if (x < 0 || y < 0 || z < 0 || x >= SX || y >= SY || z >= SZ)
+      return fail(HLB_EGEOMETRY, "hlb_create: site outside the block grid");
+    const int64_t b = ((x / bs) * BY + (y / bs)) * BZ + (z / bs);
+    if (L.blockSlot[b] < 0) {
+      L.blockSlot[b] = (int64_t)(slotInput.size() / spb);
+      slotInput.resize(slotInput.size() + spb, -1);
+    }
+    int64_t& cell = slotInput[L.blockSlot[b] * spb + (((x % bs) * bs) + (y % bs)) * bs + (z % bs)];
+    if (cell >= 0) return fail(HLB_EGEOMETRY, "hlb_create: duplicate fluid site");
+    cell = i;
+  }
+
+  // Classification of every input record.
+  std::vector<uint32_t> wall(n, 0), iol(n, 0);
+  std::vector<int32_t> ioletId(n, -1), type(n, kFluidType);
+  std::vector<uint8_t> cls(n, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    bool hadInlet = false, hadOutlet = false;
+    for (int d = 1; d < Q; ++d) {
+      const uint8_t t = g.link_type[i * 18 + d - 1];
+      if (t > HLB_LINK_OUTLET) return fail(HLB_EGEOMETRY, "hlb_create: unknown link type");
+      if (t == HLB_LINK_WALL) wall[i] |= 1u << (d - 1);
+      else if (t == HLB_LINK_INLET || t == HLB_LINK_OUTLET) iol[i] |= 1u << (d - 1);
+      if (t == HLB_LINK_INLET) {
+        ioletId[i] = g.link_iolet[i * 18 + d - 1];
+        hadInlet = true;
+      } else if (t == HLB_LINK_OUTLET) {
+        ioletId[i] = g.link_iolet[i * 18 + d - 1];
+        hadOutlet = true;
+      }
+    }
+    type[i] = hadInlet ? kInletType : (hadOutlet ? kOutletType : kFluidType);
+    const bool w = wall[i] != 0;
+    cls[i] = type[i] == kFluidType ? (w ? 1 : 0) : type[i] == kInletType ? (w ? 4 : 2) : (w ? 5 : 3);
+  }
+
+  // Class-major contiguous order.
+  int64_t count[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t i = 0; i < n; ++i) count[cls[i]]++;
+  L.off[0] = 0;
+  for (int c = 0; c < 6; ++c) L.off[c + 1] = L.off[c] + count[c];
+  L.N = n;
+  int64_t next[6];
+  for (int c = 0; c < 6; ++c) next[c] = L.off[c];
+  std::vector<int64_t> contigOf(n, -1);
+  L.slotSites.assign(slotInput.size(), -1);
+  for (int64_t b = 0; b < BX * BY * BZ; ++b) {
+    const int64_t slot = L.blockSlot[b];
+    if (slot < 0) continue;
+    for (int64_t s = 0; s < spb; ++s) {
+      const int64_t i = slotInput[slot * spb + s];
+      if (i < 0) continue;
+      const int64_t k = next[cls[i]]++;
+      contigOf[i] = k;
+      L.slotSites[slot * spb + s] = (int32_t)k;
+    }
+  }
+
+  L.coords.assign(n * 3, 0);
+  L.wall.assign(n, 0);
+  L.iolet.assign(n, 0);
+  L.ioletId.assign(n, -1);
+  L.siteType.assign(n, kFluidType);
+  L.q.assign(n * 18, -1.0f);
+  L.nrm.assign(n * 3, 0.0f);
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t k = contigOf[i];
+    for (int a = 0; a < 3; ++a) L.coords[3 * k + a] = g.coords[3 * i + a];
+    L.wall[k] = wall[i];
+    L.iolet[k] = iol[i];
+    L.ioletId[k] = ioletId[i];
+    L.siteType[k] = type[i];
+    for (int d = 0; d < 18; ++d) L.q[k * 18 + d] = g.link_dist[i * 18 + d];
+    float v[3];
+    if (g.normal_avail[i]) {
+      for (int a = 0; a < 3; ++a) v[a] = g.normal[3 * i + a];
+      normalise_float(v);
+    } else {
+      v[0] = v[1] = v[2] = std::numeric_limits<float>::infinity();  // float(NO_VALUE)
+    }
+    for (int a = 0; a < 3; ++a) L.nrm[3 * k + a] = v[a];
+  }
+
+  // Pull table and the one-writer check.
+  L.pull.assign((size_t)18 * n, 0);
+  auto linkOf = [&](int64_t k, int d) -> uint8_t {
+    // link type of contiguous site k in direction d (1..18)
+    if ((L.wall[k] >> (d - 1)) & 1u) return HLB_LINK_WALL;
+    if ((L.iolet[k] >> (d - 1)) & 1u) return HLB_LINK_INLET;
+    return HLB_LINK_NONE;
+  };
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t x = L.coords[3 * k], y = L.coords[3 * k + 1], z = L.coords[3 * k + 2];
+    for (int j = 1; j < Q; ++j) {
+      const int ib = INV[j];
+      const int64_t src = L.site_at(x - CX[j], y - CY[j], z - CZ[j]);
+      const bool plain = linkOf(k, ib) == HLB_LINK_NONE;
+      const bool srcPushes = src >= 0 && linkOf(src, j) == HLB_LINK_NONE;
+      if (plain != srcPushes) {
+        return fail(HLB_EGEOMETRY,
+                    "hlb_create: link type disagrees with neighbour fluidity at site (" + std::to_string(x) +
+                        "," + std::to_string(y) + "," + std::to_string(z) + ") direction " +
+                        std::to_string(ib) + " (the push step would write that distribution " +
+                        (plain ? "zero" : "two") + " times)");
+      }
+      L.pull[(size_t)(j - 1) * n + k] = (uint32_t)(plain ? src : k);
+    }
+  }
+  return HLB_OK;
+}
+
+void push_table(const HostLattice& L, int64_t* out) {
+  const int64_t n = L.N;
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t x = L.coords[3 * k], y = L.coords[3 * k + 1], z = L.coords[3 * k + 2];
+    out[k * Q] = k * Q;
+    for (int d = 1; d < Q; ++d) {
+      const int64_t nb = L.site_at(x + CX[d], y + CY[d], z + CZ[d]);
+      out[k * Q + d] = nb >= 0 ? nb * Q + d : n * Q;
+    }
+  }
+}
+
+}  // namespace hlb

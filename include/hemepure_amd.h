@@ -1,0 +1,231 @@
+/*
+ * hemepure_amd.h -- C ABI of the MI355X D3Q19 collide-and-stream library.
+ *
+ * The reference (HemePure) has no foreign-function interface: its hot path is a
+ * set of compile-time CRTP templates driven by lb::LBM<D3Q19>.  Every entry
+ * point below replaces one call of that host API; the comment on each names the
+ * reference interface (paths relative to the HemePure source tree).  All entry
+ * points return 0 on success and a negative HLB_E* code on failure;
+ * hlb_last_error() returns the message of the calling thread's last failure.
+ * No torch or HIP types appear in the signatures: plain pointers and sizes.
+ *
+ * Threading: a handle is single-threaded, like the reference's LBM object
+ * (one thread per MPI rank).  Work is ordered on the handle's HIP stream.
+ */
+#ifndef HEMEPURE_AMD_H
+#define HEMEPURE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define HLB_OK 0
+#define HLB_EINVAL -1    /* bad argument or configuration */
+#define HLB_EGEOMETRY -2 /* geometry the pull scheme cannot represent */
+#define HLB_EDEVICE -3   /* HIP runtime failure */
+#define HLB_ERANGE -4    /* e.g. iolet site outside a parabolic radius */
+
+/* ---- enums mirroring the reference's build-time selections ------------- */
+/* Link intersection type, GeometrySiteLink::IntersectionType
+ * (src/geometry/GeometrySiteLink.h). */
+#define HLB_LINK_NONE 0
+#define HLB_LINK_WALL 1
+#define HLB_LINK_INLET 2
+#define HLB_LINK_OUTLET 3
+
+/* HEMELB_KERNEL (src/CMakeLists.txt:47-48, src/lb/BuildSystemInterface.h:24-60). */
+#define HLB_KERNEL_LBGK 0
+#define HLB_KERNEL_TRT 1
+#define HLB_KERNEL_LES 2 /* LBGKLESSL with sponge lifetime 0 (pure Smagorinsky, C_s = 0.1) */
+
+/* HEMELB_WALL_BOUNDARY (src/lb/BuildSystemInterface.h:165-197). */
+#define HLB_WALL_SBB 0
+#define HLB_WALL_BFL 1
+
+/* HEMELB_{INLET,OUTLET}_BOUNDARY and their wall-corner variants
+ * (src/lb/BuildSystemInterface.h:215-331): Nash pressure or Ladd velocity. */
+#define HLB_IOLET_NASH 0
+#define HLB_IOLET_LADD 1
+
+/* Iolet value providers (src/lb/iolets/). */
+#define HLB_IOLET_COSINE 0    /* InOutLetCosine */
+#define HLB_IOLET_PARABOLIC 1 /* InOutLetParabolicVelocity */
+#define HLB_IOLET_FILEVEL 2   /* InOutLetFileVelocity, table already in lattice units */
+#define HLB_IOLET_WK2 3       /* InOutLetWK2 */
+#define HLB_IOLET_WK3 4       /* InOutLetWK3 */
+
+/* Storage precision of the distributions. */
+#define HLB_FP64 0
+#define HLB_FP32_SHIFTED 1
+
+/* hlb_step flags */
+#define HLB_STEP_PROPERTIES 1u /* fill rho, u and WSS during the last step */
+
+/* ---- geometry (the content of a .gmy file, decoded) --------------------- */
+/* One record per fluid site, in any order.  Directions are D3Q19 1..18
+ * (src/lb/lattices/D3Q19.cc:15-17).  Mirrors GeometrySite/GeometrySiteLink
+ * (src/geometry/GeometrySite.h, GeometrySiteLink.h) as produced by
+ * GeometryReader::ParseSite (src/geometry/GeometryReader.cc:851-935). */
+typedef struct hlb_geometry {
+  int32_t blocks[3];          /* block counts; the site grid is blocks*block_size */
+  int32_t block_size;         /* 8 in every .gmy file */
+  int64_t n_sites;
+  const int32_t* coords;      /* n*3 global site coordinates */
+  const uint8_t* link_type;   /* n*18 HLB_LINK_* */
+  const int32_t* link_iolet;  /* n*18 iolet id (-1 if none) */
+  const float* link_dist;     /* n*18 fraction of the link before the intersection */
+  const uint8_t* normal_avail;/* n */
+  const float* normal;        /* n*3 wall normal as stored (un-normalised) */
+} hlb_geometry;
+
+/* One inlet or outlet: position/normal/radius from the XML plus the
+ * provider's parameters (src/configuration/SimConfig.cc:268-400, 725-830). */
+typedef struct hlb_iolet_desc {
+  int32_t kind;            /* HLB_IOLET_COSINE .. HLB_IOLET_WK3 */
+  int32_t warmup;          /* cosine / parabolic warm-up steps */
+  double position[3];      /* lattice units */
+  double normal[3];        /* normalised in double, as InOutLet::SetNormal */
+  double radius;           /* velocity and Windkessel iolets */
+  double density_mean;     /* cosine */
+  double density_amp;      /* cosine */
+  double phase;            /* cosine, radians */
+  double period;           /* cosine, lattice steps */
+  double max_speed;        /* parabolic */
+  const double* table;     /* file velocity: max speed per 1-indexed step */
+  int64_t table_len;
+  double area;             /* Windkessel */
+  double resistance;       /* WK2 */
+  double r_c;              /* WK3 characteristic resistance */
+  double r_p;              /* WK3 peripheral resistance */
+  double capacitance;      /* WK2 / WK3 */
+} hlb_iolet_desc;
+
+/* Collision and boundary selection plus LbmParameters
+ * (src/lb/LbmParameters.h:29-45; relaxation_parameter SimConfig.cc:165-169). */
+typedef struct hlb_params {
+  int32_t kernel;     /* HLB_KERNEL_* */
+  int32_t wall_bc;    /* HLB_WALL_* */
+  int32_t inlet_bc;   /* HLB_IOLET_NASH / HLB_IOLET_LADD */
+  int32_t outlet_bc;
+  int32_t precision;  /* HLB_FP64 */
+  int32_t reserved;
+  double tau;
+  double trt_lambda;
+} hlb_params;
+
+typedef struct hlb_lattice* hlb_handle;
+
+/* ---- errors ------------------------------------------------------------- */
+/* Replaces hemelb::Exception messages (src/Exception.h:25) and the
+ * std::exit(16) of InOutLetParabolicVelocity.cc:39-45. */
+const char* hlb_last_error(void);
+
+/* ---- lifecycle ---------------------------------------------------------- */
+/* Builds the lattice and uploads it to the current HIP device.  Replaces
+ * LatticeData::LatticeData -> ProcessReadSites + InitialiseNeighbourLookups
+ * (src/geometry/LatticeData.cc:69-517), LBM::LBM + ReadParameters
+ * (src/lb/lb.hpp:361-374), BoundaryValues::BoundaryValues
+ * (src/lb/iolets/BoundaryValues.cc:20-81) and LBM::InitCollisions
+ * (src/lb/lb.hpp:73-117).  Fails with HLB_EGEOMETRY when a link's type
+ * disagrees with its neighbour's fluidity (two or zero writers of one
+ * distribution in the reference's push step). */
+int hlb_create(const hlb_geometry* geom, const hlb_params* params, const hlb_iolet_desc* inlets,
+               int n_inlets, const hlb_iolet_desc* outlets, int n_outlets, hlb_handle* out);
+/* Replaces LBM::~LBM / LatticeData::~LatticeData. */
+void hlb_destroy(hlb_handle h);
+
+/* ---- initial conditions ------------------------------------------------- */
+/* EquilibriumInitialCondition::SetFs (src/lb/InitialCondition.hpp:30-43):
+ * f = f_eq(rho0, 0) at every site; the step counter restarts at t0 (1 =
+ * SimulationState's start, src/lb/SimulationState.cc:17). */
+int hlb_init_equilibrium(hlb_handle h, double rho0, int64_t t0);
+/* CheckpointInitialCondition::SetFs (src/lb/InitialCondition.hpp:46-50,
+ * LocalDistributionInput::LoadDistribution): pre-collision f in the
+ * reference's contiguous site order, AoS f[site*19+d], double. */
+int hlb_upload_f(hlb_handle h, const double* f, int64_t t0);
+
+/* ---- the hot path ------------------------------------------------------- */
+/* Advances nsteps time steps, each one the reference's
+ * LBM::RequestComms/PreSend/PreReceive/PostReceive/EndIteration sequence
+ * (src/lb/lb.hpp:192-347) plus the BoundaryValues comms of its iolets, as
+ * fused pull-scheme collide+stream launches on the handle's stream.
+ * HLB_STEP_PROPERTIES fills density, velocity and wall-shear-stress caches
+ * during the last step (BaseStreamer::UpdateMinsAndMaxes,
+ * src/lb/streamers/BaseStreamer.h:88-120).  Asynchronous. */
+int hlb_step(hlb_handle h, int64_t nsteps, uint32_t flags);
+/* Like hlb_step, timing the launches with HIP events recorded on the handle's
+ * stream; blocks until done and returns the elapsed milliseconds. */
+int hlb_step_timed(hlb_handle h, int64_t nsteps, uint32_t flags, double* ms);
+/* Blocks until the handle's stream is idle. */
+int hlb_synchronize(hlb_handle h);
+
+/* ---- extraction / checkpoint ------------------------------------------- */
+/* LbDataSourceIterator::GetDistribution after the swap
+ * (src/extraction/LbDataSourceIterator.cc:90-93): the pre-collision f that
+ * enters the next step, AoS f[site*19+d] in the reference's site order. */
+int hlb_download_f(hlb_handle h, double* f);
+/* MacroscopicPropertyCache density/velocity/WSS of the last step run with
+ * HLB_STEP_PROPERTIES (src/lb/MacroscopicPropertyCache.h:24-106).  u is n*3;
+ * WSS is DBL_MAX (NO_VALUE) at sites without a wall link.  Any may be NULL. */
+int hlb_get_fields(hlb_handle h, double* rho, double* u, double* wss);
+/* SimulationState::GetTimeStep (src/lb/SimulationState.cc:50): the 1-indexed
+ * step that runs next. */
+int64_t hlb_timestep(hlb_handle h);
+/* StabilityTester (src/lb/StabilityTester.h:100-117): 1 if every
+ * distribution entering the next step is > 0 (and not NaN). */
+int hlb_stability(hlb_handle h, int* stable);
+
+/* ---- lattice tables (LatticeData accessors, for parity and extraction) -- */
+int64_t hlb_site_count(hlb_handle h);          /* GetLocalFluidSiteCount */
+/* Get{MidDomain,DomainEdge}CollisionCount (LatticeData.h:375-436). */
+int hlb_class_counts(hlb_handle h, int64_t out[6]);
+/* GetGlobalSiteCoords (LatticeData.h:611-614), n*3. */
+int hlb_site_coords(hlb_handle h, int64_t* out);
+/* The reference's push table GetStreamedIndex (LatticeData.h:516-520,
+ * LatticeData.cc:366-443): n*19 int64, rubbish slot n*19, rebuilt from the
+ * device pull table.  pull may be NULL; else it receives the device pull
+ * table (18 rows of n uint32: source site of direction j = 1..18, or the
+ * site itself where a boundary condition supplies the value). */
+int hlb_neighbour_tables(hlb_handle h, int64_t* push, uint32_t* pull);
+/* SiteData masks, ioletId and SiteType (src/geometry/SiteDataBare.cc:24-76). */
+int hlb_site_masks(hlb_handle h, uint32_t* wall, uint32_t* iolet, int32_t* iolet_id,
+                   int32_t* site_type);
+/* GetCutDistance (n*18, double) and GetNormalToWall (n*3, double). */
+int hlb_wall_geometry(hlb_handle h, double* dist, double* normal);
+
+/* ---- procedural geometries (the inputs a .gmy file would hold) ---------- */
+typedef struct hlb_geom hlb_geom;
+typedef struct hlb_iolet_geom {
+  int32_t is_inlet;
+  int32_t id;
+  double position[3];
+  double normal[3]; /* points into the fluid */
+  double radius;
+} hlb_iolet_geom;
+
+/* Straight voxelised cylinder along z: radius r, length L sites, axis at the
+ * centre of a box_xy x box_xy cross-section; inlet at z = -0.5, outlet at
+ * z = L - 0.5. */
+int hlb_geom_cylinder(double radius, int32_t length, int32_t box_xy, hlb_geom** out);
+/* Symmetric Y-bifurcation: parent (radius rp, length lp) along +z, two
+ * daughters (radius rd, length ld) at +-angle_deg in the x-z plane; radii
+ * carry a seeded smooth jitter of relative amplitude jitter. */
+int hlb_geom_ybifurcation(double rp, double lp, double rd, double ld, double angle_deg,
+                          double jitter, uint64_t seed, hlb_geom** out);
+/* Straight pipe with a Gaussian stenosis of the given area reduction at
+ * mid-length, width parameter w (lattice units). */
+int hlb_geom_stenosed_pipe(double radius, int32_t length, double area_reduction, double width,
+                           hlb_geom** out);
+/* Fills the view with pointers into g (valid until hlb_geom_free). */
+int hlb_geom_view(const hlb_geom* g, hlb_geometry* view);
+int hlb_geom_iolets(const hlb_geom* g, hlb_iolet_geom* out, int max_iolets);
+void hlb_geom_free(hlb_geom* g);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HEMEPURE_AMD_H */

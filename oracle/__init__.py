@@ -1,0 +1,270 @@
+"""ctypes binding of the CPU oracle (oracle/hlb_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+bench.py's cpu_baseline leg, always as the checker, never as the product path.
+The oracle is a plain double-precision restatement of HemePure's push-scheme
+D3Q19 step (see the header of hlb_oracle.cpp for the reference files it
+follows).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "libhlb_oracle.so"
+Q = 19
+
+
+class OrcIolet(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("warmup", C.c_int32),
+        ("position", C.c_double * 3),
+        ("normal", C.c_double * 3),
+        ("radius", C.c_double),
+        ("density_mean", C.c_double),
+        ("density_amp", C.c_double),
+        ("phase", C.c_double),
+        ("period", C.c_double),
+        ("max_speed", C.c_double),
+        ("table", C.POINTER(C.c_double)),
+        ("table_len", C.c_int64),
+        ("area", C.c_double),
+        ("resistance", C.c_double),
+        ("r_c", C.c_double),
+        ("r_p", C.c_double),
+        ("capacitance", C.c_double),
+    ]
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("kernel", C.c_int32),
+        ("wall_bc", C.c_int32),
+        ("inlet_bc", C.c_int32),
+        ("outlet_bc", C.c_int32),
+        ("sse3_order", C.c_int32),
+        ("pad", C.c_int32),
+        ("tau", C.c_double),
+        ("trt_lambda", C.c_double),
+    ]
+
+
+_lib = None
+vp = C.c_void_p
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        sigs = {
+            "orc_lattice_build": (vp, [C.POINTER(C.c_int32), C.c_int32, C.c_int64, vp, vp, vp, vp, vp, vp,
+                                       C.c_char_p, C.c_int]),
+            "orc_lattice_free": (None, [vp]),
+            "orc_site_count": (C.c_int64, [vp]),
+            "orc_class_counts": (None, [vp, vp]),
+            "orc_site_coords": (None, [vp, vp]),
+            "orc_push_table": (None, [vp, vp]),
+            "orc_site_masks": (None, [vp, vp, vp, vp, vp]),
+            "orc_wall_distances": (None, [vp, vp]),
+            "orc_wall_normals": (None, [vp, vp]),
+            "orc_lbm_create": (vp, [vp, C.POINTER(OrcParams), C.POINTER(OrcIolet), C.c_int,
+                                    C.POINTER(OrcIolet), C.c_int, C.c_char_p, C.c_int]),
+            "orc_lbm_free": (None, [vp]),
+            "orc_init_equilibrium": (None, [vp, C.c_double]),
+            "orc_set_f": (None, [vp, vp]),
+            "orc_run": (C.c_int, [vp, C.c_int64, C.c_int, C.c_int, C.c_char_p, C.c_int]),
+            "orc_get_f": (None, [vp, vp]),
+            "orc_get_props": (None, [vp, vp, vp, vp]),
+            "orc_timestep": (C.c_int64, [vp]),
+            "orc_iolet_state": (None, [vp, C.c_int, C.c_int, vp]),
+            "orc_stable": (C.c_int, [vp]),
+            "orc_moments": (None, [vp, C.c_int, vp]),
+            "orc_feq": (None, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, vp]),
+            "orc_collide": (None, [C.c_int, C.c_double, C.c_double, vp, C.c_int, vp, vp]),
+            "orc_wss": (C.c_double, [C.c_double, C.c_double, vp, vp, vp]),
+            "orc_cosine_density": (C.c_double, [C.POINTER(OrcIolet), C.c_double, C.c_uint64]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+def iolet(kind, position, normal, radius=0.0, density_mean=1.0, density_amp=0.0, phase=0.0, period=1.0,
+          warmup=0, max_speed=0.0, table=None, area=1.0, resistance=1.0, r_c=1.0, r_p=1.0,
+          capacitance=1.0) -> OrcIolet:
+    d = OrcIolet()
+    d.kind, d.warmup = kind, warmup
+    d.position[:] = tuple(position)
+    d.normal[:] = tuple(normal)
+    d.radius, d.density_mean, d.density_amp, d.phase, d.period = radius, density_mean, density_amp, phase, period
+    d.max_speed = max_speed
+    if table is not None:
+        t = np.ascontiguousarray(table, dtype=np.float64)
+        d._keep = t
+        d.table = t.ctypes.data_as(C.POINTER(C.c_double))
+        d.table_len = t.size
+    d.area, d.resistance, d.r_c, d.r_p, d.capacitance = area, resistance, r_c, r_p, capacitance
+    return d
+
+
+class OracleLattice:
+    """LatticeData of the reference, restated on the CPU."""
+
+    def __init__(self, geom):
+        err = C.create_string_buffer(512)
+        blocks = (C.c_int32 * 3)(*geom.blocks)
+        self._keep = geom
+        h = lib().orc_lattice_build(blocks, geom.block_size, geom.n_sites, _p(geom.coords), _p(geom.link_type),
+                                    _p(geom.link_iolet), _p(geom.link_dist), _p(geom.normal_avail),
+                                    _p(geom.normal), err, 512)
+        if not h:
+            raise RuntimeError(err.value.decode())
+        self._h = h
+        self.n = int(lib().orc_site_count(h))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_lattice_free(self._h)
+            self._h = None
+
+    def class_counts(self):
+        out = np.zeros(6, dtype=np.int64)
+        lib().orc_class_counts(self._h, _p(out))
+        return out
+
+    def site_coords(self):
+        out = np.zeros((self.n, 3), dtype=np.int64)
+        lib().orc_site_coords(self._h, _p(out))
+        return out
+
+    def push_table(self):
+        out = np.zeros((self.n, Q), dtype=np.int64)
+        lib().orc_push_table(self._h, _p(out))
+        return out
+
+    def site_masks(self):
+        w = np.zeros(self.n, dtype=np.uint32)
+        i = np.zeros(self.n, dtype=np.uint32)
+        ident = np.zeros(self.n, dtype=np.int32)
+        t = np.zeros(self.n, dtype=np.int32)
+        lib().orc_site_masks(self._h, _p(w), _p(i), _p(ident), _p(t))
+        return w, i, ident, t
+
+    def wall_distances(self):
+        out = np.zeros((self.n, 18), dtype=np.float64)
+        lib().orc_wall_distances(self._h, _p(out))
+        return out
+
+    def wall_normals(self):
+        out = np.zeros((self.n, 3), dtype=np.float64)
+        lib().orc_wall_normals(self._h, _p(out))
+        return out
+
+
+class OracleLBM:
+    """LBM<D3Q19> plus BoundaryValues of the reference, restated on the CPU."""
+
+    def __init__(self, lat: OracleLattice, kernel=0, wall_bc=0, inlet_bc=0, outlet_bc=0, tau=0.8,
+                 trt_lambda=3.0 / 16.0, inlets=(), outlets=(), sse3_order=True):
+        self.lat = lat
+        p = OrcParams()
+        p.kernel, p.wall_bc, p.inlet_bc, p.outlet_bc = kernel, wall_bc, inlet_bc, outlet_bc
+        p.sse3_order = 1 if sse3_order else 0
+        p.tau, p.trt_lambda = tau, trt_lambda
+        ins = (OrcIolet * max(len(inlets), 1))(*inlets)
+        outs = (OrcIolet * max(len(outlets), 1))(*outlets)
+        self._keep = (list(inlets), list(outlets))
+        err = C.create_string_buffer(512)
+        h = lib().orc_lbm_create(lat._h, C.byref(p), ins, len(inlets), outs, len(outlets), err, 512)
+        if not h:
+            raise RuntimeError(err.value.decode())
+        self._h = h
+        self.n = lat.n
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_lbm_free(self._h)
+            self._h = None
+
+    def init_equilibrium(self, rho0=1.0):
+        lib().orc_init_equilibrium(self._h, rho0)
+
+    def set_f(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(self.n, Q)
+        lib().orc_set_f(self._h, _p(f))
+
+    def run(self, nsteps, properties=False, threads=1):
+        err = C.create_string_buffer(512)
+        rc = lib().orc_run(self._h, nsteps, 1 if properties else 0, threads, err, 512)
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+
+    def f(self):
+        out = np.zeros((self.n, Q), dtype=np.float64)
+        lib().orc_get_f(self._h, _p(out))
+        return out
+
+    def fields(self):
+        rho = np.zeros(self.n)
+        u = np.zeros((self.n, 3))
+        wss = np.zeros(self.n)
+        lib().orc_get_props(self._h, _p(rho), _p(u), _p(wss))
+        return rho, u, wss
+
+    def timestep(self):
+        return int(lib().orc_timestep(self._h))
+
+    def stable(self):
+        return bool(lib().orc_stable(self._h))
+
+
+def moments(f, sse3=True):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(4)
+    lib().orc_moments(_p(f), 1 if sse3 else 0, _p(out))
+    return out
+
+
+def feq(rho, mx, my, mz, sse3=True):
+    out = np.zeros(Q)
+    lib().orc_feq(rho, mx, my, mz, 1 if sse3 else 0, _p(out))
+    return out
+
+
+def collide(kernel, tau, lam, f, sse3=True):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(Q)
+    t = np.zeros(1)
+    lib().orc_collide(kernel, tau, lam, _p(f), 1 if sse3 else 0, _p(out), _p(t))
+    return out, float(t[0])
+
+
+def wss(rho, tau, fpost, f, n):
+    fpost = np.ascontiguousarray(fpost, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    return float(lib().orc_wss(rho, tau, _p(fpost), _p(f), _p(n)))
+
+
+def cosine_density(io: OrcIolet, min_density: float, t0: int) -> float:
+    return float(lib().orc_cosine_density(C.byref(io), min_density, t0))

@@ -1,0 +1,867 @@
+// hlb_oracle.cpp -- CPU oracle for the D3Q19 collide-and-stream step.
+//
+// TEST INFRASTRUCTURE ONLY (the checker, never the product).  This file restates,
+// in plain double-precision C++, what HemePure computes for one rank:
+//   * LatticeData construction: site classification, class-major contiguous
+//     ordering, the int64 push table with its rubbish slot, wall distances and
+//     wall normals (src/geometry/SiteDataBare.cc, src/geometry/LatticeData.cc,
+//     src/geometry/LatticeData.h);
+//   * the D3Q19 moments and equilibrium in either the SSE3 or the scalar
+//     evaluation order (src/lb/lattices/Lattice.h);
+//   * LBGK, TRT and LBGK+LES kernels (src/lb/kernels/*.h);
+//   * the streamer factories with SimpleBounceBack, BFL, Nash zeroth-order
+//     pressure and Ladd velocity delegates (src/lb/streamers/*.h);
+//   * cosine, parabolic/file velocity and Windkessel iolets (src/lb/iolets/*.cc);
+//   * the LBM phase sequence PreReceive -> PostReceive -> EndIteration and the
+//     BoundaryValues comms step (src/lb/lb.hpp:192-347).
+// Compiled with -ffp-contract=off so every product and sum is rounded exactly
+// where the C++ source rounds it.  Paths cited below are relative to the
+// HemePure source tree.
+
+#include "hlb_oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr int Q = 19;
+
+// D3Q19 tables, src/lb/lattices/D3Q19.cc:15-33.
+const int CX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+const int CY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+const int CZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+const double CXD[Q] = {0.0, 1.0, -1.0, 0.0, 0.0, 0.0, 0.0, 1.0, -1.0, 1.0,
+                       -1.0, 1.0, -1.0, 1.0, -1.0, 0.0, 0.0, 0.0, 0.0};
+const double CYD[Q] = {0.0, 0.0, 0.0, 1.0, -1.0, 0.0, 0.0, 1.0, -1.0, -1.0,
+                       1.0, 0.0, 0.0, 0.0, 0.0, 1.0, -1.0, 1.0, -1.0};
+const double CZD[Q] = {0.0, 0.0, 0.0, 0.0, 0.0, 1.0, -1.0, 0.0, 0.0, 0.0,
+                       0.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0};
+const double W[Q] = {1.0 / 3.0,  1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0,
+                     1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                     1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                     1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+const int INV[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+
+// src/constants.h:54,61 and src/lb/lattices/LatticeInfo.h site types.
+const double Cs2 = 1.0 / 3.0;
+const double NO_VALUE = DBL_MAX;
+const double PI = 3.14159265358979323846264338327950288;
+enum { SOLID_TYPE = 0, FLUID_TYPE = 1, INLET_TYPE = 2, OUTLET_TYPE = 3 };
+
+void set_err(char* err, int errlen, const std::string& msg) {
+  if (err && errlen > 0) {
+    std::snprintf(err, (size_t)errlen, "%s", msg.c_str());
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Lattice formulas.
+
+// Density and momentum, SSE3 path, src/lb/lattices/Lattice.h:51-110: the odd
+// (19th) element seeds the high lane, pairs (d, d+1) accumulate in lanes
+// (low, high), and the horizontal add returns low + high.
+inline void MomentsSSE3(const double* f, double& rho, double& mx, double& my, double& mz) {
+  double r0 = 0.0, r1 = f[18];
+  double x0 = 0.0, x1 = CXD[18] * f[18];
+  double y0 = 0.0, y1 = CYD[18] * f[18];
+  double z0 = 0.0, z1 = CZD[18] * f[18];
+  for (int d = 0; d < 18; d += 2) {
+    r0 = r0 + f[d];
+    r1 = r1 + f[d + 1];
+    x0 = x0 + CXD[d] * f[d];
+    x1 = x1 + CXD[d + 1] * f[d + 1];
+    y0 = y0 + CYD[d] * f[d];
+    y1 = y1 + CYD[d + 1] * f[d + 1];
+    z0 = z0 + CZD[d] * f[d];
+    z1 = z1 + CZD[d + 1] * f[d + 1];
+  }
+  rho = r0 + r1;
+  mx = x0 + x1;
+  my = y0 + y1;
+  mz = z0 + z1;
+}
+
+// Scalar path, src/lb/lattices/Lattice.h:315-329 (int CX promoted per term).
+inline void MomentsScalar(const double* f, double& rho, double& mx, double& my, double& mz) {
+  rho = mx = my = mz = 0.0;
+  for (int d = 0; d < Q; ++d) {
+    rho += f[d];
+    mx += CX[d] * f[d];
+    my += CY[d] * f[d];
+    mz += CZ[d] * f[d];
+  }
+}
+
+// Equilibrium, SSE3 path, src/lb/lattices/Lattice.h:374-448 (the 19th element
+// takes the scalar tail, which evaluates the same expression).
+inline void FeqSSE3(double rho, double mx, double my, double mz, double* feq) {
+  const double tHMS = (3. / 2.) * (mx * mx + my * my + mz * mz);
+  const double d1 = 1. / rho;
+  const double nineHalfD1 = (9. / 2.) * d1;
+  for (int i = 0; i < Q; ++i) {
+    const double mde = (CXD[i] * mx + CYD[i] * my) + CZD[i] * mz;
+    const double tmp1 = rho - tHMS * d1;
+    const double tmp2 = nineHalfD1 * (mde * mde);
+    const double tmp3 = 3. * mde;
+    feq[i] = W[i] * ((tmp1 + tmp2) + tmp3);
+  }
+}
+
+// Scalar path, src/lb/lattices/Lattice.h:643-660.
+inline void FeqScalar(double rho, double mx, double my, double mz, double* feq) {
+  const double d1 = 1. / rho;
+  const double mms = mx * mx + my * my + mz * mz;
+  for (int i = 0; i < Q; ++i) {
+    const double mde = CX[i] * mx + CY[i] * my + CZ[i] * mz;
+    feq[i] = W[i] * (rho - (3. / 2.) * mms * d1 + (9. / 2.) * d1 * mde * mde + 3. * mde);
+  }
+}
+
+inline void Moments(bool sse3, const double* f, double& rho, double& mx, double& my, double& mz) {
+  if (sse3) MomentsSSE3(f, rho, mx, my, mz); else MomentsScalar(f, rho, mx, my, mz);
+}
+inline void Feq(bool sse3, double rho, double mx, double my, double mz, double* feq) {
+  if (sse3) FeqSSE3(rho, mx, my, mz, feq); else FeqScalar(rho, mx, my, mz, feq);
+}
+
+// Smagorinsky relaxation time, src/lb/kernels/LBGKLESSpongeLayer.h:150-182
+// (dx = dt = rho1 = 1, C_smag = 0.1 hard-coded).
+inline double TauSmagorinsky(double tau0, const double* fneq) {
+  const double dx = 1.0, dt = 1.0, C = dx / dt, rho1 = 1.0, C_smag = 0.1;
+  const int* cv[3] = {CX, CY, CZ};
+  double Q_12 = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) {
+      double qij = 0.0;
+      for (int v = 0; v < Q; ++v) qij += cv[i][v] * cv[j][v] * fneq[v];
+      Q_12 += qij * qij;
+    }
+  }
+  Q_12 = std::sqrt(Q_12);
+  return 1. / 2. *
+         (tau0 + std::sqrt((tau0 * rho1 * C) * (tau0 * rho1 * C) +
+                           18.0 * 1.4142135623730950488016887242097 * rho1 * C_smag * C_smag * Q_12) /
+                     (rho1 * C));
+}
+
+// Collision on one site.  LBGK: src/lb/kernels/LBGK.h:64-72; TRT:
+// src/lb/kernels/TRT.h:80-109 (pairs (0,0), (1,2), (3,4), ... in the order the
+// constructor builds them at :43-53); LES: LBGKLESSpongeLayer.h:70-79 with the
+// per-site tau from CalculateTau (:122-148; lifetime 0 selects tau_les).
+struct Hydro {
+  double rho, mx, my, mz, ux, uy, uz, tau;
+  double feq[Q], fneq[Q], fpost[Q];
+};
+
+inline void PreCollision(bool sse3, int kernel, double tau0, const double* f, Hydro& h) {
+  // HydroVars.tau is set from the parameters before the collider runs
+  // (src/lb/streamers/StreamerTypeFactory.h:66).
+  h.tau = tau0;
+  Moments(sse3, f, h.rho, h.mx, h.my, h.mz);
+  // Lattice.h:933-948
+  h.ux = h.mx / h.rho;
+  h.uy = h.my / h.rho;
+  h.uz = h.mz / h.rho;
+  Feq(sse3, h.rho, h.mx, h.my, h.mz, h.feq);
+  for (int i = 0; i < Q; ++i) h.fneq[i] = f[i] - h.feq[i];
+  if (kernel == ORC_KERNEL_LES) h.tau = TauSmagorinsky(tau0, h.fneq);
+}
+
+inline void Collide(int kernel, double tau0, double omega, double lambda, const double* f, Hydro& h) {
+  if (kernel == ORC_KERNEL_LBGK) {
+    for (int d = 0; d < Q; ++d) h.fpost[d] = f[d] + h.fneq[d] * omega;
+  } else if (kernel == ORC_KERNEL_LES) {
+    const double om = -1.0 / h.tau;
+    for (int d = 0; d < Q; ++d) h.fpost[d] = f[d] + h.fneq[d] * om;
+  } else {
+    const double tau_plus = tau0;
+    const double omega_plus = omega;
+    const double tau_minus = 0.5 + lambda / (tau_plus - 0.5);
+    const double omega_minus = -1.0 / tau_minus;
+    h.fpost[0] = f[0] + omega_plus * h.fneq[0];
+    for (int i = 0; i < Q; ++i) {
+      const int ib = INV[i];
+      if (ib < i) continue;
+      const double sym = 0.5 * omega_plus * (h.fneq[i] + h.fneq[ib]);
+      const double asym = 0.5 * omega_minus * (h.fneq[i] - h.fneq[ib]);
+      h.fpost[i] = f[i] + sym + asym;
+      h.fpost[ib] = f[ib] + sym - asym;
+    }
+  }
+}
+
+// Wall shear stress magnitude, src/lb/lattices/Lattice.h:1122-1156 with the
+// Pi tensor of :1353-1383.
+inline double WallShearStress(double tau, const double* fpost, const double* f, const double nor[3]) {
+  const int* cv[3] = {CX, CY, CZ};
+  double pi[3][3];
+  for (int ii = 0; ii < 3; ++ii) {
+    for (int jj = 0; jj <= ii; ++jj) {
+      pi[ii][jj] = 0.0;
+      for (int l = 0; l < Q; ++l) pi[ii][jj] += (fpost[l] - f[l]) * cv[ii][l] * cv[jj][l];
+      pi[ii][jj] *= -(tau - 0.5);
+    }
+  }
+  for (int ii = 0; ii < 3; ++ii)
+    for (int jj = ii + 1; jj < 3; ++jj) pi[ii][jj] = pi[jj][ii];
+  double sv[3] = {0.0, 0.0, 0.0};
+  double square = 0.0, normal = 0.0;
+  for (int i = 0; i < 3; i++) {
+    for (int j = 0; j < 3; j++) sv[i] += pi[i][j] * nor[j];
+    square += sv[i] * sv[i];
+    normal += sv[i] * nor[i];
+  }
+  return std::sqrt(square - normal * normal);
+}
+
+// ---------------------------------------------------------------------------
+// Lattice data.
+
+struct SiteData {  // src/geometry/SiteDataBare.cc:24-76
+  uint32_t wall = 0, iolet = 0;
+  int32_t ioletId = -1;
+  int32_t type = SOLID_TYPE;
+  bool HasWall(int d) const { return d > 0 && (wall & (1u << (d - 1))) != 0; }
+  bool HasIolet(int d) const { return d > 0 && (iolet & (1u << (d - 1))) != 0; }
+  // src/geometry/SiteDataBare.cc:107-150 + LatticeData.cc:220-240.
+  int CollisionClass() const {
+    if (wall == 0) return type == FLUID_TYPE ? 0 : (type == INLET_TYPE ? 2 : 3);
+    return type == FLUID_TYPE ? 1 : (type == INLET_TYPE ? 4 : 5);
+  }
+};
+
+}  // namespace
+
+struct orc_lattice {
+  int32_t blocks[3];
+  int32_t bs;
+  int64_t N = 0;
+  int64_t classCount[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<SiteData> site;
+  std::vector<double> dist;    // N*18, float promoted (LatticeData.h:404-405,642)
+  std::vector<double> normal;  // N*3, float-normalised then promoted (LatticeData.cc:242-244)
+  std::vector<int64_t> coords; // N*3 (LatticeData.h:410)
+  std::vector<int64_t> nbr;    // N*19 push table (LatticeData.cc:366-443)
+};
+
+namespace {
+
+// Vector3D<float>::GetNormalised, src/util/Vector3D.h:237-265 in float.
+void NormaliseFloat(float& x, float& y, float& z) {
+  if (x == 0.0f && y == 0.0f && z == 0.0f) return;
+  const float mag = std::sqrt(x * x + y * y + z * z);
+  x /= mag;
+  y /= mag;
+  z /= mag;
+}
+
+}  // namespace
+
+extern "C" orc_lattice* orc_lattice_build(const int32_t blocks[3], int32_t bs, int64_t n,
+                                          const int32_t* coords, const uint8_t* link_type,
+                                          const int32_t* link_iolet, const float* link_dist,
+                                          const uint8_t* normal_avail, const float* normal,
+                                          char* err, int errlen) {
+  auto* L = new orc_lattice();
+  for (int a = 0; a < 3; ++a) L->blocks[a] = blocks[a];
+  L->bs = bs;
+  const int64_t BX = blocks[0], BY = blocks[1], BZ = blocks[2];
+  const int64_t nblocks = BX * BY * BZ;
+  const int64_t spb = (int64_t)bs * bs * bs;
+  const int64_t SX = BX * bs, SY = BY * bs, SZ = BZ * bs;
+
+  // Blocks holding at least one fluid site are the ones a .gmy file lists
+  // (GeometryReader); every other block reads as solid.
+  std::vector<int64_t> blockSlot(nblocks, -1);
+  std::vector<int64_t> siteInput;  // per present block, spb entries: input index or -1
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t x = coords[3 * i], y = coords[3 * i + 1], z = coords[3 * i + 2];
+    if (x < 0 || y < 0 || z < 0 || x >= SX || y >= SY || z >= SZ) {
+      set_err(err, errlen, "site outside the block grid");
+      delete L;
+      return nullptr;
+    }
+    const int64_t b = ((x / bs) * BY + (y / bs)) * BZ + (z / bs);
+    if (blockSlot[b] < 0) {
+      blockSlot[b] = (int64_t)(siteInput.size() / spb);
+      siteInput.resize(siteInput.size() + spb, -1);
+    }
+    const int64_t s = (((x % bs) * bs) + (y % bs)) * bs + (z % bs);
+    if (siteInput[blockSlot[b] * spb + s] >= 0) {
+      set_err(err, errlen, "duplicate fluid site");
+      delete L;
+      return nullptr;
+    }
+    siteInput[blockSlot[b] * spb + s] = i;
+  }
+
+  // Classify each fluid site, src/geometry/SiteDataBare.cc:24-76.
+  std::vector<SiteData> sd(n);
+  for (int64_t i = 0; i < n; ++i) {
+    SiteData& s = sd[i];
+    bool hadInlet = false, hadOutlet = false;
+    for (int d = 1; d < Q; ++d) {
+      const int t = link_type[i * 18 + d - 1];
+      if (t == ORC_LINK_WALL) {
+        s.wall |= 1u << (d - 1);
+      } else if (t == ORC_LINK_INLET || t == ORC_LINK_OUTLET) {
+        s.iolet |= 1u << (d - 1);
+      }
+      if (t == ORC_LINK_INLET) {
+        s.ioletId = link_iolet[i * 18 + d - 1];
+        hadInlet = true;
+      } else if (t == ORC_LINK_OUTLET) {
+        s.ioletId = link_iolet[i * 18 + d - 1];
+        hadOutlet = true;
+      }
+    }
+    s.type = hadInlet ? INLET_TYPE : (hadOutlet ? OUTLET_TYPE : FLUID_TYPE);
+  }
+
+  // Class-major ordering over the block traversal (z fastest) and the in-block
+  // site traversal (z fastest), src/geometry/LatticeData.cc:69-285,
+  // LatticeData.h:375-436, src/geometry/VolumeTraverser.cc:56-70.  With one
+  // rank every site is mid-domain.
+  std::vector<int64_t> order[6];
+  for (int64_t b = 0; b < nblocks; ++b) {
+    if (blockSlot[b] < 0) continue;
+    for (int64_t s = 0; s < spb; ++s) {
+      const int64_t i = siteInput[blockSlot[b] * spb + s];
+      if (i < 0) continue;
+      order[sd[i].CollisionClass()].push_back(i);
+    }
+  }
+  std::vector<int64_t> contig(n, -1);
+  int64_t N = 0;
+  for (int c = 0; c < 6; ++c) {
+    L->classCount[c] = (int64_t)order[c].size();
+    for (int64_t i : order[c]) contig[i] = N++;
+  }
+  L->N = N;
+  L->site.resize(N);
+  L->dist.resize(N * 18);
+  L->normal.resize(N * 3);
+  L->coords.resize(N * 3);
+  for (int c = 0; c < 6; ++c) {
+    for (int64_t i : order[c]) {
+      const int64_t k = contig[i];
+      L->site[k] = sd[i];
+      for (int d = 0; d < 18; ++d) L->dist[k * 18 + d] = (double)link_dist[i * 18 + d];
+      float nx, ny, nz;
+      if (normal_avail[i]) {
+        nx = normal[3 * i];
+        ny = normal[3 * i + 1];
+        nz = normal[3 * i + 2];
+        NormaliseFloat(nx, ny, nz);
+      } else {
+        // util::Vector3D<float>(NO_VALUE): DBL_MAX narrowed to float is +inf.
+        nx = ny = nz = std::numeric_limits<float>::infinity();
+      }
+      L->normal[k * 3] = (double)nx;
+      L->normal[k * 3 + 1] = (double)ny;
+      L->normal[k * 3 + 2] = (double)nz;
+      for (int a = 0; a < 3; ++a) L->coords[k * 3 + a] = coords[3 * i + a];
+    }
+  }
+
+  // Push table, src/geometry/LatticeData.cc:366-443: d = 0 maps to itself;
+  // neighbours outside the grid or solid go to the rubbish slot N*19.
+  L->nbr.assign(N * Q, -1);
+  for (int64_t k = 0; k < N; ++k) {
+    const int64_t x = L->coords[3 * k], y = L->coords[3 * k + 1], z = L->coords[3 * k + 2];
+    L->nbr[k * Q] = k * Q;
+    for (int d = 1; d < Q; ++d) {
+      const int64_t nx = x + CX[d], ny = y + CY[d], nz = z + CZ[d];
+      int64_t target = N * Q;
+      if (nx >= 0 && ny >= 0 && nz >= 0 && nx < SX && ny < SY && nz < SZ) {
+        const int64_t b = ((nx / bs) * BY + (ny / bs)) * BZ + (nz / bs);
+        if (blockSlot[b] >= 0) {
+          const int64_t s = (((nx % bs) * bs) + (ny % bs)) * bs + (nz % bs);
+          const int64_t i = siteInput[blockSlot[b] * spb + s];
+          if (i >= 0) target = contig[i] * Q + d;
+        }
+      }
+      L->nbr[k * Q + d] = target;
+    }
+  }
+  (void)err;
+  return L;
+}
+
+extern "C" void orc_lattice_free(orc_lattice* l) { delete l; }
+extern "C" int64_t orc_site_count(const orc_lattice* l) { return l->N; }
+extern "C" void orc_class_counts(const orc_lattice* l, int64_t out[6]) {
+  for (int c = 0; c < 6; ++c) out[c] = l->classCount[c];
+}
+extern "C" void orc_site_coords(const orc_lattice* l, int64_t* out) {
+  std::memcpy(out, l->coords.data(), sizeof(int64_t) * l->coords.size());
+}
+extern "C" void orc_push_table(const orc_lattice* l, int64_t* out) {
+  std::memcpy(out, l->nbr.data(), sizeof(int64_t) * l->nbr.size());
+}
+extern "C" void orc_site_masks(const orc_lattice* l, uint32_t* wall, uint32_t* iolet, int32_t* iolet_id,
+                               int32_t* site_type) {
+  for (int64_t k = 0; k < l->N; ++k) {
+    wall[k] = l->site[k].wall;
+    iolet[k] = l->site[k].iolet;
+    iolet_id[k] = l->site[k].ioletId;
+    site_type[k] = l->site[k].type;
+  }
+}
+extern "C" void orc_wall_distances(const orc_lattice* l, double* out) {
+  std::memcpy(out, l->dist.data(), sizeof(double) * l->dist.size());
+}
+extern "C" void orc_wall_normals(const orc_lattice* l, double* out) {
+  std::memcpy(out, l->normal.data(), sizeof(double) * l->normal.size());
+}
+
+// ---------------------------------------------------------------------------
+// Iolets.
+
+namespace {
+
+struct Iolet {
+  orc_iolet d;
+  double normal[3];          // InOutLet::SetNormal normalises in double (InOutLet.h:169-172)
+  std::vector<double> table; // file velocity table
+  double minSimDensity = 0.0;
+  int64_t centreSite = -1;   // BoundaryValues.cc:109-128
+  int dirs0 = 0;             // first of the directions sorted by closeness to the normal
+  // Windkessel state, InOutLetWK2.cc:13-18 / InOutLetWK3.cc:13-18.
+  double density = 1.0, densityNew = 1.0;
+  double flowRateOld = 0.0, flowRate = 0.0, flowRateNew = 0.0;
+  int64_t siteCount = 0;
+  bool IsWK() const { return d.kind == ORC_IOLET_WK2 || d.kind == ORC_IOLET_WK3; }
+  bool IsVelocity() const { return d.kind == ORC_IOLET_PARABOLIC || d.kind == ORC_IOLET_FILEVEL; }
+  double DensityMin() const {
+    if (d.kind == ORC_IOLET_COSINE) return d.density_mean - d.density_amp;  // InOutLetCosine.h:46-49
+    return 1.0;  // velocity and Windkessel iolets
+  }
+};
+
+// InOutLetCosine::GetDensity, src/lb/iolets/InOutLetCosine.cc:37-54.
+double CosineDensity(const orc_iolet& d, double minSimDensity, uint64_t time_step) {
+  const double w = 2.0 * PI / d.period;
+  const double target = d.density_mean + d.density_amp * std::cos(w * time_step + d.phase);
+  if (time_step >= (uint64_t)(uint32_t)d.warmup) return target;
+  const double interpolationFactor = ((double)time_step) / ((double)(uint32_t)d.warmup);
+  return interpolationFactor * target + (1. - interpolationFactor) * minSimDensity;
+}
+
+// BoundaryValues::GetBoundaryDensity (BoundaryValues.cc:211-214): the
+// 0-indexed step; Windkessel and velocity iolets return their state / 1.0.
+double BoundaryDensity(const Iolet& io, uint64_t t0) {
+  switch (io.d.kind) {
+    case ORC_IOLET_COSINE: return CosineDensity(io.d, io.minSimDensity, t0);
+    case ORC_IOLET_WK2:
+    case ORC_IOLET_WK3: return io.density;
+    default: return 1.0;
+  }
+}
+
+// Parabolic / file velocity at x, src/lb/iolets/InOutLetParabolicVelocity.cc:28-58
+// and InOutLetFileVelocity.cc:85-101 (table branch).  t is the 1-indexed step.
+bool IoletVelocity(const Iolet& io, const double x[3], uint64_t t, double v[3], std::string& msg) {
+  const double dx = x[0] - io.d.position[0], dy = x[1] - io.d.position[1], dz = x[2] - io.d.position[2];
+  const double z = dx * io.normal[0] + dy * io.normal[1] + dz * io.normal[2];
+  const double rSq = (dx * dx + dy * dy + dz * dz) - z * z;
+  const double rSqOverASq = rSq / (io.d.radius * io.d.radius);
+  if (rSqOverASq > 1.0) {
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "An IOLET site with r = %lf lies outside the IOLET radius %lf.",
+                  std::sqrt(rSq), io.d.radius);
+    msg = buf;
+    return false;
+  }
+  double max;
+  if (io.d.kind == ORC_IOLET_PARABOLIC) {
+    max = io.d.max_speed;
+    if (t < (uint64_t)(uint32_t)io.d.warmup) max *= t / double((uint32_t)io.d.warmup);
+  } else {
+    if ((int64_t)t >= (int64_t)io.table.size()) {
+      msg = "velocity table shorter than the run";
+      return false;
+    }
+    max = io.table[t];
+  }
+  const double s = max * (1. - rSqOverASq);
+  v[0] = io.normal[0] * s;
+  v[1] = io.normal[1] * s;
+  v[2] = io.normal[2] * s;
+  return true;
+}
+
+// Windkessel DoPreStreamCoupling centre-site update (InOutLetWK2.cc:82-95,
+// InOutLetWK3.cc:70-95); the flow-rate accumulation is done by the caller in
+// site order.
+void WKCentreUpdate(Iolet& io) {
+  const double pressure = io.density * Cs2;  // GetPressure(0)
+  if (io.d.kind == ORC_IOLET_WK2) {
+    const double R0 = io.d.resistance, C0 = io.d.capacitance;
+    const double pressureNew = R0 / (1.0 + R0 * C0) * (io.flowRate + C0 * pressure);
+    io.densityNew = pressureNew / Cs2;
+  } else {
+    const double Rc = io.d.r_c, Rp = io.d.r_p, Cp = io.d.capacitance;
+    const double pressureNew = (Rp * Cp * pressure + (Rc + Rp) * (2.0 * io.flowRate - io.flowRateOld) +
+                                Rc * Rp * Cp * (io.flowRate - io.flowRateOld)) /
+                               (1.0 + Rp * Cp);
+    io.densityNew = pressureNew / Cs2;
+  }
+}
+
+// DirectionsCloseToNormal, NashZerothOrderPressureDelegate.h:95-116 (the same
+// std::sort over the same comparator).
+int SortDirections(const double n[3]) {
+  std::array<int, Q> dirs;
+  std::array<double, Q> dist;
+  for (int k = 0; k < Q; ++k) {
+    double cx = CXD[k], cy = CYD[k], cz = CZD[k];
+    if (!(cx == 0.0 && cy == 0.0 && cz == 0.0)) {
+      const double mag = std::sqrt(cx * cx + cy * cy + cz * cz);
+      cx /= mag;
+      cy /= mag;
+      cz /= mag;
+    }
+    const double ex = cx - n[0], ey = cy - n[1], ez = cz - n[2];
+    dist[k] = ex * ex + ey * ey + ez * ez;
+    dirs[k] = k;
+  }
+  std::sort(dirs.begin(), dirs.end(), [&dist](int i, int j) { return dist[i] < dist[j]; });
+  return dirs[0];
+}
+
+}  // namespace
+
+struct orc_lbm {
+  orc_lattice* L;
+  orc_params p;
+  double omega;
+  std::vector<Iolet> inlets, outlets;
+  std::vector<double> fOld, fNew;
+  std::vector<double> rho, u, wss;  // property caches (MacroscopicPropertyCache)
+  std::vector<double> wkContrib;    // per-site WK flux contribution this step
+  std::vector<uint8_t> wkFlag;
+  uint64_t timeStep = 1;            // SimulationState starts at 1 (SimulationState.cc:17)
+  bool stable = true;
+};
+
+extern "C" orc_lbm* orc_lbm_create(orc_lattice* l, const orc_params* p, const orc_iolet* inl, int n_in,
+                                   const orc_iolet* outl, int n_out, char* err, int errlen) {
+  auto* m = new orc_lbm();
+  m->L = l;
+  m->p = *p;
+  m->omega = -1.0 / p->tau;  // LbmParameters.h:36-37
+  const int64_t N = l->N;
+  m->fOld.assign(N * Q + 1, 0.0);
+  m->fNew.assign(N * Q + 1, 0.0);
+  m->rho.assign(N, 0.0);
+  m->u.assign(N * 3, 0.0);
+  m->wss.assign(N, 0.0);
+  m->wkContrib.assign(N, 0.0);
+  m->wkFlag.assign(N, 0);
+  auto setup = [&](std::vector<Iolet>& v, const orc_iolet* src, int cnt) {
+    for (int i = 0; i < cnt; ++i) {
+      Iolet io;
+      io.d = src[i];
+      const double* nn = src[i].normal;
+      double nx = nn[0], ny = nn[1], nz = nn[2];
+      if (!(nx == 0.0 && ny == 0.0 && nz == 0.0)) {
+        const double mag = std::sqrt(nx * nx + ny * ny + nz * nz);
+        nx /= mag;
+        ny /= mag;
+        nz /= mag;
+      }
+      io.normal[0] = nx;
+      io.normal[1] = ny;
+      io.normal[2] = nz;
+      if (src[i].kind == ORC_IOLET_FILEVEL && src[i].table)
+        io.table.assign(src[i].table, src[i].table + src[i].table_len);
+      io.dirs0 = SortDirections(io.normal);
+      // Centre site: first site within +-1 of the position in every axis
+      // (BoundaryValues.cc:109-128, NumericalFunctions::IsInRange inclusive).
+      for (int64_t k = 0; k < N; ++k) {
+        bool in = true;
+        for (int a = 0; a < 3; ++a) {
+          const double c = (double)l->coords[3 * k + a];
+          in = in && (c >= io.d.position[a] - 1.0) && (c <= io.d.position[a] + 1.0);
+        }
+        if (in) {
+          io.centreSite = k;
+          break;
+        }
+      }
+      v.push_back(io);
+    }
+  };
+  setup(m->inlets, inl, n_in);
+  setup(m->outlets, outl, n_out);
+  // PrepareBoundaryObjects, src/lb/lb.hpp:119-143.
+  double minDensity = DBL_MAX;
+  for (auto& io : m->inlets) minDensity = std::min(minDensity, io.DensityMin());
+  for (auto& io : m->outlets) minDensity = std::min(minDensity, io.DensityMin());
+  for (auto& io : m->inlets) io.minSimDensity = minDensity;
+  for (auto& io : m->outlets) io.minSimDensity = minDensity;
+  (void)err;
+  (void)errlen;
+  return m;
+}
+
+extern "C" void orc_lbm_free(orc_lbm* m) { delete m; }
+
+// EquilibriumInitialCondition::SetFs, src/lb/InitialCondition.hpp:30-43.
+extern "C" void orc_init_equilibrium(orc_lbm* m, double rho0) {
+  double feq[Q];
+  Feq(m->p.sse3_order != 0, rho0, 0.0, 0.0, 0.0, feq);
+  const int64_t N = m->L->N;
+  for (int64_t k = 0; k < N; ++k)
+    for (int d = 0; d < Q; ++d) m->fOld[k * Q + d] = m->fNew[k * Q + d] = feq[d];
+}
+
+extern "C" void orc_set_f(orc_lbm* m, const double* f) {
+  const int64_t N = m->L->N;
+  for (int64_t i = 0; i < N * Q; ++i) m->fOld[i] = m->fNew[i] = f[i];
+}
+
+namespace {
+
+// One site of StreamAndCollide for its class, following the three factories
+// of src/lb/streamers/StreamerTypeFactory.h:48-83 (wall), 134-169 (iolet),
+// 232-276 (wall + iolet) and SimpleCollideAndStream.h:39-68 (bulk).
+bool StreamSite(orc_lbm* m, int cls, int64_t k, bool props, std::string& msg) {
+  const orc_lattice* L = m->L;
+  const SiteData& s = L->site[k];
+  const bool sse3 = m->p.sse3_order != 0;
+  const double* f = &m->fOld[k * Q];
+  double* fNew = m->fNew.data();
+  Hydro h;
+  PreCollision(sse3, m->p.kernel, m->p.tau, f, h);
+  Collide(m->p.kernel, m->p.tau, m->omega, m->p.trt_lambda, f, h);
+  const bool inletSide = (cls == 2 || cls == 4);
+  std::vector<Iolet>& iolets = inletSide ? m->inlets : m->outlets;
+  const int ioletBC = inletSide ? m->p.inlet_bc : m->p.outlet_bc;
+
+  for (int d = 0; d < Q; ++d) {
+    const bool useIolet = (cls >= 2) && s.HasIolet(d);
+    const bool useWall = !useIolet && (cls == 1 || cls >= 4) && s.HasWall(d);
+    const int inv = INV[d];
+    if (useIolet) {
+      Iolet& io = iolets[s.ioletId];
+      const int unstreamed = inv;
+      if (ioletBC == ORC_IOLET_NASH) {
+        // NashZerothOrderPressureDelegate::StreamLink, :36-76.
+        if (unstreamed == io.dirs0 && io.IsWK()) {
+          m->wkFlag[k] = 1;  // DoPreStreamCoupling: flux term accumulated in site order
+          m->wkContrib[k] = h.ux * -io.normal[0] + h.uy * -io.normal[1] + h.uz * -io.normal[2];
+        }
+        const double ghostDensity = BoundaryDensity(io, m->timeStep - 1);
+        const float nfx = (float)io.normal[0], nfy = (float)io.normal[1], nfz = (float)io.normal[2];
+        const double vx = h.mx / h.rho, vy = h.my / h.rho, vz = h.mz / h.rho;
+        const double component = vx * (double)nfx + vy * (double)nfy + vz * (double)nfz;
+        const double gx = (double)nfx * component * ghostDensity;
+        const double gy = (double)nfy * component * ghostDensity;
+        const double gz = (double)nfz * component * ghostDensity;
+        double geq[Q];
+        Feq(sse3, ghostDensity, gx, gy, gz, geq);
+        fNew[k * Q + unstreamed] = geq[unstreamed];
+      } else {
+        // LaddIoletDelegate::StreamLink, src/lb/streamers/LaddIoletDelegate.h:37-86.
+        double pos[3] = {(double)L->coords[3 * k] + 0.5 * CX[d], (double)L->coords[3 * k + 1] + 0.5 * CY[d],
+                         (double)L->coords[3 * k + 2] + 0.5 * CZ[d]};
+        double wm[3];
+        if (!IoletVelocity(io, pos, m->timeStep, wm, msg)) return false;
+        wm[0] *= h.rho;
+        wm[1] *= h.rho;
+        wm[2] *= h.rho;
+        const double correction = 2. * W[d] * (wm[0] * CX[d] + wm[1] * CY[d] + wm[2] * CZ[d]) / Cs2;
+        fNew[k * Q + inv] = h.fpost[d] - correction;
+      }
+    } else if (useWall) {
+      if (m->p.wall_bc == ORC_WALL_SBB) {
+        // SimpleBounceBackDelegate::StreamLink, :36-44.
+        fNew[k * Q + inv] = h.fpost[d];
+      } else {
+        // BouzidiFirdaousLallemandDelegate::StreamLink, :44-70.
+        const double q = L->dist[k * 18 + d - 1];
+        if (s.HasWall(inv) || q < 0.5) {
+          fNew[k * Q + inv] = h.fpost[d];
+        } else {
+          fNew[k * Q + inv] = (h.fpost[d] + (2.0 * q - 1) * h.fpost[inv]) / (2.0 * q);
+        }
+      }
+    } else {
+      // SimpleCollideAndStreamDelegate::StreamLink, :28-36.
+      fNew[L->nbr[k * Q + d]] = h.fpost[d];
+    }
+  }
+
+  if (props) {
+    // BaseStreamer::UpdateMinsAndMaxes, src/lb/streamers/BaseStreamer.h:88-120.
+    m->rho[k] = h.rho;
+    m->u[3 * k] = h.ux;
+    m->u[3 * k + 1] = h.uy;
+    m->u[3 * k + 2] = h.uz;
+    if (s.wall == 0) {
+      m->wss[k] = NO_VALUE;
+    } else {
+      m->wss[k] = WallShearStress(h.tau, h.fpost, f, &L->normal[3 * k]);
+    }
+  }
+  return true;
+}
+
+// Post-step for one site: BFL completion (BouzidiFirdaousLallemandDelegate.h:72-93)
+// and the iolet post-stream coupling (Nash :78-89, Ladd :88-101), in the
+// direction order of the factories' DoPostStep.
+void PostStepSite(orc_lbm* m, int cls, int64_t k) {
+  const orc_lattice* L = m->L;
+  const SiteData& s = L->site[k];
+  double* fn = &m->fNew[k * Q];
+  const bool inletSide = (cls == 2 || cls == 4);
+  std::vector<Iolet>& iolets = inletSide ? m->inlets : m->outlets;
+  for (int d = 0; d < Q; ++d) {
+    const bool wallPost = (cls == 1 || cls >= 4) && s.HasWall(d);
+    const bool ioletPost = !wallPost && cls >= 2 && s.HasIolet(d);
+    if (wallPost && m->p.wall_bc == ORC_WALL_BFL) {
+      const int inv = INV[d];
+      const double q = L->dist[k * 18 + d - 1];
+      if (!s.HasWall(inv) && q < 0.5) fn[inv] = 2.0 * q * fn[inv] + (1.0 - 2.0 * q) * fn[d];
+    } else if (ioletPost) {
+      Iolet& io = iolets[s.ioletId];
+      if (INV[d] == io.dirs0 && io.IsWK() && k == io.centreSite) {
+        io.density = io.densityNew;  // DoPostStreamCoupling
+        io.flowRateOld = io.flowRate;
+      }
+    }
+  }
+}
+
+int64_t ClassStart(const orc_lattice* L, int c) {
+  int64_t o = 0;
+  for (int i = 0; i < c; ++i) o += L->classCount[i];
+  return o;
+}
+
+}  // namespace
+
+extern "C" int orc_run(orc_lbm* m, int64_t nsteps, int props_last, int nthreads, char* err, int errlen) {
+  const orc_lattice* L = m->L;
+  const int64_t N = L->N;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  for (int64_t step = 0; step < nsteps; ++step) {
+    const bool props = props_last && step == nsteps - 1;
+    // BeginPhase: BoundaryValues::RequestComms -> InOutLetWK*::DoComms
+    // (InOutLetWK3.cc:33-52): density <- densityNew (self-send on one rank),
+    // flowRate <- last step's accumulation scaled by area / siteCount.
+    for (auto* v : {&m->inlets, &m->outlets}) {
+      for (auto& io : *v) {
+        if (!io.IsWK()) continue;
+        io.density = io.densityNew;
+        io.flowRate = io.flowRateNew;
+        if (io.siteCount != 0) io.flowRate = io.flowRate * io.d.area / io.siteCount;
+        io.flowRateNew = 0.0;
+        io.siteCount = 0;
+      }
+    }
+    std::fill(m->wkFlag.begin(), m->wkFlag.end(), 0);
+    // PreReceive: the six classes in order (src/lb/lb.hpp:243-278).
+    bool ok = true;
+    std::string msg;
+    for (int c = 0; c < 6; ++c) {
+      const int64_t first = ClassStart(L, c), count = L->classCount[c];
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+      for (int64_t k = first; k < first + count; ++k) {
+        std::string lmsg;
+        if (!StreamSite(m, c, k, props, lmsg)) {
+          ok = false;
+#pragma omp critical
+          msg = lmsg;
+        }
+      }
+    }
+    if (!ok) {
+      set_err(err, errlen, msg);
+      return -1;
+    }
+    // DoPreStreamCoupling of the Windkessel iolets, in site order: the centre
+    // site recomputes densityNew from this step's density and flow rates, and
+    // every coupling site adds u.(-n) to flowRateNew.
+    for (int64_t k = ClassStart(L, 2); k < N; ++k) {
+      if (!m->wkFlag[k]) continue;
+      const int c = L->site[k].CollisionClass();
+      Iolet& io = (c == 2 || c == 4) ? m->inlets[L->site[k].ioletId] : m->outlets[L->site[k].ioletId];
+      if (k == io.centreSite) WKCentreUpdate(io);
+      io.flowRateNew += m->wkContrib[k];
+      io.siteCount++;
+    }
+    // PostReceive: post-step on every class (src/lb/lb.hpp:280-334).
+    for (int c = 1; c < 6; ++c) {
+      const int64_t first = ClassStart(L, c), count = L->classCount[c];
+      for (int64_t k = first; k < first + count; ++k) PostStepSite(m, c, k);
+    }
+    // StabilityTester: any !(f > 0) in fNew is unstable (StabilityTester.h:100-117).
+    bool st = true;
+    for (int64_t i = 0; i < N * Q && st; ++i) st = m->fNew[i] > 0.0;
+    m->stable = st;
+    // EndIteration: swap (src/lb/lb.hpp:336-347), then SimulationState::Increment.
+    std::swap(m->fOld, m->fNew);
+    m->timeStep++;
+  }
+  return 0;
+}
+
+extern "C" void orc_get_f(const orc_lbm* m, double* out) {
+  std::memcpy(out, m->fOld.data(), sizeof(double) * m->L->N * Q);
+}
+extern "C" void orc_get_props(const orc_lbm* m, double* rho, double* u, double* wss) {
+  const int64_t N = m->L->N;
+  if (rho) std::memcpy(rho, m->rho.data(), sizeof(double) * N);
+  if (u) std::memcpy(u, m->u.data(), sizeof(double) * N * 3);
+  if (wss) std::memcpy(wss, m->wss.data(), sizeof(double) * N);
+}
+extern "C" int64_t orc_timestep(const orc_lbm* m) { return (int64_t)m->timeStep; }
+extern "C" void orc_iolet_state(const orc_lbm* m, int outlet, int i, double out[4]) {
+  const Iolet& io = outlet ? m->outlets[i] : m->inlets[i];
+  out[0] = io.IsWK() ? io.density : BoundaryDensity(io, m->timeStep - 2);
+  out[1] = io.densityNew;
+  out[2] = io.flowRate;
+  out[3] = io.flowRateOld;
+}
+extern "C" int orc_stable(const orc_lbm* m) { return m->stable ? 1 : 0; }
+
+extern "C" void orc_moments(const double* f, int sse3, double out[4]) {
+  Moments(sse3 != 0, f, out[0], out[1], out[2], out[3]);
+}
+extern "C" void orc_feq(double rho, double mx, double my, double mz, int sse3, double* feq) {
+  Feq(sse3 != 0, rho, mx, my, mz, feq);
+}
+extern "C" void orc_collide(int kernel, double tau, double lambda, const double* f, int sse3, double* fpost,
+                            double* tau_out) {
+  Hydro h;
+  PreCollision(sse3 != 0, kernel, tau, f, h);
+  Collide(kernel, tau, -1.0 / tau, lambda, f, h);
+  std::memcpy(fpost, h.fpost, sizeof h.fpost);
+  if (tau_out) *tau_out = h.tau;
+}
+extern "C" double orc_wss(double rho, double tau, const double* fpost, const double* f, const double n[3]) {
+  (void)rho;
+  return WallShearStress(tau, fpost, f, n);
+}
+extern "C" double orc_cosine_density(const orc_iolet* io, double min_density, uint64_t t0) {
+  return CosineDensity(*io, min_density, t0);
+}

@@ -1,0 +1,134 @@
+/*
+ * hlb_oracle.h -- C API of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is a plain C++ restatement, in double
+ * precision, of HemePure's D3Q19 collide-and-stream step exactly as the
+ * reference computes it: push scheme, AoS distributions f[site*19+d], an int64
+ * push table with a rubbish slot, the six collision classes advanced in
+ * class order, the post-step pass and the old/new swap.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+ * only as the checker; the product library never links or calls it.
+ *
+ * Every function of the implementation cites the reference file:line it
+ * restates (paths relative to the HemePure source tree).
+ */
+#ifndef HLB_ORACLE_H
+#define HLB_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Link intersection types, as GeometrySiteLink::IntersectionType
+ * (src/geometry/GeometrySiteLink.h). */
+#define ORC_LINK_NONE 0
+#define ORC_LINK_WALL 1
+#define ORC_LINK_INLET 2
+#define ORC_LINK_OUTLET 3
+
+/* Collision kernels (HEMELB_KERNEL, src/CMakeLists.txt:47-48). */
+#define ORC_KERNEL_LBGK 0
+#define ORC_KERNEL_TRT 1
+#define ORC_KERNEL_LES 2 /* LBGKLESSL with lifetime 0: pure Smagorinsky LES */
+
+/* Wall BCs (HEMELB_WALL_BOUNDARY). */
+#define ORC_WALL_SBB 0
+#define ORC_WALL_BFL 1
+
+/* Iolet BCs (HEMELB_INLET_BOUNDARY / HEMELB_OUTLET_BOUNDARY). */
+#define ORC_IOLET_NASH 0
+#define ORC_IOLET_LADD 1
+
+/* Iolet value providers. */
+#define ORC_IOLET_COSINE 0     /* InOutLetCosine (pressure) */
+#define ORC_IOLET_PARABOLIC 1  /* InOutLetParabolicVelocity */
+#define ORC_IOLET_FILEVEL 2    /* InOutLetFileVelocity with a given lattice-unit table */
+#define ORC_IOLET_WK2 3        /* InOutLetWK2 */
+#define ORC_IOLET_WK3 4        /* InOutLetWK3 */
+
+typedef struct orc_iolet {
+  int32_t kind;
+  int32_t warmup;          /* cosine / parabolic warm-up length in steps */
+  double position[3];      /* lattice units */
+  double normal[3];        /* normalised by the oracle, as InOutLet::SetNormal */
+  double radius;           /* velocity and Windkessel iolets */
+  double density_mean;     /* cosine */
+  double density_amp;      /* cosine */
+  double phase;            /* cosine */
+  double period;           /* cosine, lattice steps */
+  double max_speed;        /* parabolic */
+  const double* table;     /* file velocity: max speed per 1-indexed step */
+  int64_t table_len;
+  double area;             /* Windkessel */
+  double resistance;       /* WK2 R */
+  double r_c;              /* WK3 characteristic resistance */
+  double r_p;              /* WK3 peripheral resistance */
+  double capacitance;      /* WK2/WK3 C */
+} orc_iolet;
+
+typedef struct orc_params {
+  int32_t kernel;
+  int32_t wall_bc;
+  int32_t inlet_bc;
+  int32_t outlet_bc;
+  int32_t sse3_order;  /* 1: moment/f_eq arithmetic order of the SSE3 build
+                          (Lattice.h:51-110, 374-448); 0: scalar (315-329, 643-660) */
+  int32_t pad;
+  double tau;
+  double trt_lambda;   /* relaxation_parameter */
+} orc_params;
+
+typedef struct orc_lattice orc_lattice;
+typedef struct orc_lbm orc_lbm;
+
+/* Build LatticeData from per-fluid-site geometry records (any order).
+ * coords: n*3 global site coordinates; link_*: n*18 records for D3Q19
+ * directions 1..18; normal_avail/normal: wall normal as read (un-normalised).
+ * Returns NULL on error with a message in err. */
+orc_lattice* orc_lattice_build(const int32_t blocks[3], int32_t block_size, int64_t n,
+                               const int32_t* coords, const uint8_t* link_type,
+                               const int32_t* link_iolet, const float* link_dist,
+                               const uint8_t* normal_avail, const float* normal,
+                               char* err, int errlen);
+void orc_lattice_free(orc_lattice* l);
+int64_t orc_site_count(const orc_lattice* l);
+void orc_class_counts(const orc_lattice* l, int64_t out[6]);
+void orc_site_coords(const orc_lattice* l, int64_t* out);      /* n*3 */
+void orc_push_table(const orc_lattice* l, int64_t* out);       /* n*19 */
+void orc_site_masks(const orc_lattice* l, uint32_t* wall, uint32_t* iolet, int32_t* iolet_id,
+                    int32_t* site_type);
+void orc_wall_distances(const orc_lattice* l, double* out);    /* n*18 */
+void orc_wall_normals(const orc_lattice* l, double* out);      /* n*3 */
+
+orc_lbm* orc_lbm_create(orc_lattice* l, const orc_params* p, const orc_iolet* inlets, int n_in,
+                        const orc_iolet* outlets, int n_out, char* err, int errlen);
+void orc_lbm_free(orc_lbm* m);
+void orc_init_equilibrium(orc_lbm* m, double rho0);
+void orc_set_f(orc_lbm* m, const double* f);  /* fOld = fNew = f (AoS n*19) */
+/* Advance nsteps.  props_last: compute rho/u/WSS during the last step.
+ * nthreads: OpenMP threads (1 = sequential).  Returns 0, or -1 on an error
+ * (message in err) such as an iolet site outside a parabolic radius. */
+int orc_run(orc_lbm* m, int64_t nsteps, int props_last, int nthreads, char* err, int errlen);
+void orc_get_f(const orc_lbm* m, double* out);  /* pre-collision f entering the next step */
+void orc_get_props(const orc_lbm* m, double* rho, double* u, double* wss);
+int64_t orc_timestep(const orc_lbm* m);          /* 1-indexed step about to run */
+/* Iolet state: density used by the step that just ran, plus WK flow rates. */
+void orc_iolet_state(const orc_lbm* m, int outlet, int i, double out[4]);
+/* Stability: 1 if every f of the last fNew is > 0 (StabilityTester.h:100-117). */
+int orc_stable(const orc_lbm* m);
+
+/* Standalone lattice formulas for closed-form and golden checks. */
+void orc_moments(const double* f, int sse3, double out[4]);
+void orc_feq(double rho, double mx, double my, double mz, int sse3, double* feq);
+void orc_collide(int kernel, double tau, double lambda, const double* f, int sse3, double* fpost,
+                 double* tau_out);
+double orc_wss(double rho, double tau, const double* fpost, const double* f, const double n[3]);
+double orc_cosine_density(const orc_iolet* io, double min_density, uint64_t t0);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
