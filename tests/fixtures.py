@@ -1,0 +1,180 @@
+"""Shared test fixtures: small geometries and case builders.
+
+A *case* names a geometry, the collision/BC selection and the iolet
+conditions once, and builds both the product handle (hemepure_amd.Lattice)
+and the oracle (oracle.OracleLBM) from it, so every parity test runs both on
+identical inputs.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+import hemepure_amd as H
+
+# D3Q19 vectors, src/lb/lattices/D3Q19.cc:15-17.
+CX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0])
+CY = np.array([0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1])
+CZ = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1])
+INV = np.array([0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17])
+W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+
+
+def four_cube(sites_per_block: int = 6, seed: int = 7) -> H.Geometry:
+    """The reference's FourCube fixture (src/unittests/FourCubeLatticeData.h:61-167)
+    restated for D3Q19 with seeded cut distances: a (n-2)^3 fluid cube in one
+    n^3 block; z = min face is inlet 0, z = max face outlet 0, x/y faces walls;
+    a link leaving through z counts as the iolet even at edges."""
+    rng = np.random.default_rng(seed)
+    n = sites_per_block
+    lo, hi = 1, n - 2
+    coords, lt, li, ld, na, nrm = [], [], [], [], [], []
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                if not (lo <= i <= hi and lo <= j <= hi and lo <= k <= hi):
+                    continue
+                coords.append((i, j, k))
+                t_row, i_row, d_row = [], [], []
+                for d in range(1, 19):
+                    ni, nj, nk = i + CX[d], j + CY[d], k + CZ[d]
+                    dist = float(rng.integers(0, 10000)) / 10000.0
+                    if nk < lo:
+                        t_row.append(H.LINK_INLET); i_row.append(0); d_row.append(dist)
+                    elif nk > hi:
+                        t_row.append(H.LINK_OUTLET); i_row.append(0); d_row.append(dist)
+                    elif ni < lo or nj < lo or ni > hi or nj > hi:
+                        t_row.append(H.LINK_WALL); i_row.append(-1); d_row.append(dist)
+                    else:
+                        t_row.append(H.LINK_NONE); i_row.append(-1); d_row.append(-1.0)
+                lt.append(t_row); li.append(i_row); ld.append(d_row)
+                nv = None
+                if i == lo:
+                    nv = (-1, 0, 0)
+                if i == hi:
+                    nv = (1, 0, 0)
+                if j == lo:
+                    nv = (0, -1, 0)
+                if j == hi:
+                    nv = (0, 1, 0)
+                na.append(1 if nv else 0)
+                nrm.append(nv or (0, 0, 0))
+    iolets = [dict(is_inlet=True, id=0, position=(0.5 * (n - 1), 0.5 * (n - 1), lo - 0.5), normal=(0, 0, 1),
+                   radius=float(n)),
+              dict(is_inlet=False, id=0, position=(0.5 * (n - 1), 0.5 * (n - 1), hi + 0.5), normal=(0, 0, -1),
+                   radius=float(n))]
+    return H.Geometry((1, 1, 1), np.array(coords), np.array(lt), np.array(li), np.array(ld, dtype=np.float32),
+                      np.array(na), np.array(nrm, dtype=np.float32), iolets, block_size=n)
+
+
+def closed_box(nx: int, ny: int, nz: int, seed: int = 3) -> H.Geometry:
+    """A box of fluid with wall links on every face (no iolets); seeded q."""
+    rng = np.random.default_rng(seed)
+    bs = 8
+    blocks = [math.ceil((v + 2) / bs) for v in (nx, ny, nz)]
+    xs, ys, zs = np.meshgrid(np.arange(1, nx + 1), np.arange(1, ny + 1), np.arange(1, nz + 1), indexing="ij")
+    coords = np.stack([xs.ravel(), ys.ravel(), zs.ravel()], axis=1)
+    n = coords.shape[0]
+    lt = np.zeros((n, 18), dtype=np.uint8)
+    ld = np.full((n, 18), -1.0, dtype=np.float32)
+    for d in range(1, 19):
+        nb = coords + np.array([CX[d], CY[d], CZ[d]])
+        out = (nb[:, 0] < 1) | (nb[:, 0] > nx) | (nb[:, 1] < 1) | (nb[:, 1] > ny) | (nb[:, 2] < 1) | (nb[:, 2] > nz)
+        lt[out, d - 1] = H.LINK_WALL
+        ld[out, d - 1] = rng.uniform(0.05, 0.95, size=int(out.sum())).astype(np.float32)
+    li = np.full((n, 18), -1, dtype=np.int32)
+    na = (lt == H.LINK_WALL).any(axis=1).astype(np.uint8)
+    nrm = np.zeros((n, 3), dtype=np.float32)
+    nrm[na == 1] = (0.3, 0.4, 0.5)
+    return H.Geometry(tuple(blocks), coords, lt, li, ld, na, nrm, [])
+
+
+def anisotropic_f(n: int) -> np.ndarray:
+    """LbTestsHelper::InitialiseAnisotropicTestData (src/unittests/lbtests/LbTestsHelper.h:193-199)."""
+    d = np.arange(19)
+    s = np.arange(n)[:, None]
+    return (d[None, :] + 1) / 10.0 + s / 100.0
+
+
+@dataclass
+class IoletSpec:
+    kind: int
+    position: tuple
+    normal: tuple
+    radius: float = 0.0
+    density_mean: float = 1.0
+    density_amp: float = 0.0
+    phase: float = 0.0
+    period: float = 1.0
+    warmup: int = 0
+    max_speed: float = 0.0
+    table: np.ndarray | None = None
+
+    def kw(self):
+        return dict(position=self.position, normal=self.normal, radius=self.radius,
+                    density_mean=self.density_mean, density_amp=self.density_amp, phase=self.phase,
+                    period=self.period, warmup=self.warmup, max_speed=self.max_speed, table=self.table)
+
+
+@dataclass
+class Case:
+    geom: H.Geometry
+    kernel: int = H.KERNEL_LBGK
+    wall_bc: int = H.WALL_SBB
+    inlet_bc: int = H.IOLET_NASH
+    outlet_bc: int = H.IOLET_NASH
+    tau: float = 0.8
+    trt_lambda: float = 3.0 / 16.0
+    inlets: list = field(default_factory=list)
+    outlets: list = field(default_factory=list)
+
+    def product(self) -> H.Lattice:
+        prm = H.params(kernel=self.kernel, wall_bc=self.wall_bc, inlet_bc=self.inlet_bc,
+                       outlet_bc=self.outlet_bc, tau=self.tau, trt_lambda=self.trt_lambda)
+        ins = [H.iolet_desc(s.kind, **s.kw()) for s in self.inlets]
+        outs = [H.iolet_desc(s.kind, **s.kw()) for s in self.outlets]
+        return H.Lattice(self.geom, prm, ins, outs)
+
+    def oracle(self):
+        import oracle as O
+        lat = O.OracleLattice(self.geom)
+        ins = [O.iolet(s.kind, **s.kw()) for s in self.inlets]
+        outs = [O.iolet(s.kind, **s.kw()) for s in self.outlets]
+        return O.OracleLBM(lat, kernel=self.kernel, wall_bc=self.wall_bc, inlet_bc=self.inlet_bc,
+                           outlet_bc=self.outlet_bc, tau=self.tau, trt_lambda=self.trt_lambda, inlets=ins,
+                           outlets=outs)
+
+
+def cylinder_case(radius=10.0, length=100, box_xy=24, tau=0.8, rho_in=1.001, rho_out=0.999, amp=0.0,
+                  period=1.0, kernel=H.KERNEL_LBGK, wall_bc=H.WALL_SBB) -> Case:
+    """The straight-cylinder configuration: LBGK, SBB walls, Nash cosine iolets."""
+    g = H.Geometry.cylinder(radius, length, box_xy)
+    i0, o0 = g.inlets()[0], g.outlets()[0]
+    return Case(g, kernel=kernel, wall_bc=wall_bc, tau=tau,
+                inlets=[IoletSpec(H.COSINE, i0["position"], i0["normal"], density_mean=rho_in, density_amp=amp,
+                                  period=period)],
+                outlets=[IoletSpec(H.COSINE, o0["position"], o0["normal"], density_mean=rho_out, period=period)])
+
+
+def sine_velocity_table(nsteps: int, peak: float, period: float) -> np.ndarray:
+    """Peak inlet speed per 1-indexed step for a pulsatile inflow with the given
+    peak and period, 0.5 peak (1 - cos(2 pi t / period)); the table layout of
+    InOutLetFileVelocity::CalculateTable (InOutLetFileVelocity.cc:36-80)."""
+    t = np.arange(nsteps + 1, dtype=np.float64)
+    return 0.5 * peak * (1.0 - np.cos(2.0 * np.pi * t / period))
+
+
+def bifurcation_case(rp=24.0, lp=200.0, rd=19.0, ld=160.0, steps=8000, peak=0.05, period=4000.0,
+                     tau=0.6, seed=2026) -> Case:
+    """The Y-bifurcation configuration: TRT (magic 3/16), BFL walls, Ladd
+    parabolic inlet driven by a sinusoidal speed table, two Nash outlets."""
+    g = H.Geometry.ybifurcation(rp, lp, rd, ld, 30.0, 0.03, seed)
+    i0 = g.inlets()[0]
+    table = sine_velocity_table(steps + 2, peak, period)
+    outs = [IoletSpec(H.COSINE, o["position"], o["normal"], density_mean=1.0) for o in g.outlets()]
+    return Case(g, kernel=H.KERNEL_TRT, wall_bc=H.WALL_BFL, inlet_bc=H.IOLET_LADD, outlet_bc=H.IOLET_NASH,
+                tau=tau, trt_lambda=3.0 / 16.0,
+                inlets=[IoletSpec(H.FILEVEL, i0["position"], i0["normal"], radius=i0["radius"], table=table)],
+                outlets=outs)

@@ -1,0 +1,77 @@
+"""The C-ABI library loads, exports every entry point include/hemepure_amd.h
+declares, and rejects bad configurations with the reference's messages.
+No device work: these run on the CPU."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import hemepure_amd as H
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "hemepure_amd.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hlb_\w+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert len(names) >= 20
+    dll = ctypes.CDLL(str(H.LIB_PATH))
+    missing = [n for n in names if not hasattr(dll, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts_match_header_sizes():
+    # 4*int32 + int64 + 6 pointers
+    assert ctypes.sizeof(H.GeometryView) == 16 + 8 + 6 * 8
+    assert ctypes.sizeof(H.Params) == 6 * 4 + 2 * 8
+    assert ctypes.sizeof(H.IoletDesc) == 8 + 6 * 8 + 6 * 8 + 8 + 8 + 5 * 8
+
+
+def _cyl():
+    return H.Geometry.cylinder(6.0, 20, 16)
+
+
+def test_null_and_bad_arguments():
+    L = H.lib()
+    h = ctypes.c_void_p()
+    assert L.hlb_create(None, None, None, 0, None, 0, ctypes.byref(h)) == H.HlbError(-1, "").code
+    g = _cyl()
+    with pytest.raises(H.HlbError, match="tau must exceed"):
+        H.Lattice(g, H.params(tau=0.5), [H.iolet_desc(H.COSINE, (0, 0, 0), (0, 0, 1))],
+                  [H.iolet_desc(H.COSINE, (0, 0, 0), (0, 0, -1))])
+    with pytest.raises(H.HlbError, match="names iolet"):
+        H.Lattice(g, H.params(), [], [H.iolet_desc(H.COSINE, (0, 0, 0), (0, 0, -1))])
+
+
+def test_ladd_needs_velocity_iolet():
+    g = _cyl()
+    with pytest.raises(H.HlbError, match="velocity iolet"):
+        H.Lattice(g, H.params(inlet_bc=H.IOLET_LADD), [H.iolet_desc(H.COSINE, (0, 0, 0), (0, 0, 1))],
+                  [H.iolet_desc(H.COSINE, (0, 0, 0), (0, 0, -1))])
+
+
+def test_parabolic_radius_violation_is_reported():
+    """InOutLetParabolicVelocity.cc:39-45 stops the reference with exit(16)."""
+    g = _cyl()
+    i0, o0 = g.inlets()[0], g.outlets()[0]
+    with pytest.raises(H.HlbError, match="lies outside the IOLET radius") as e:
+        H.Lattice(g, H.params(inlet_bc=H.IOLET_LADD),
+                  [H.iolet_desc(H.PARABOLIC, i0["position"], i0["normal"], radius=2.0, max_speed=0.01)],
+                  [H.iolet_desc(H.COSINE, o0["position"], o0["normal"])])
+    assert e.value.code == -4
+
+
+def test_host_tables_without_device():
+    g = _cyl()
+    lat = H.Lattice(g, H.params(), [H.iolet_desc(H.COSINE, (0, 0, 0), (0, 0, 1))],
+                    [H.iolet_desc(H.COSINE, (0, 0, 0), (0, 0, -1))])
+    assert lat.n == g.n_sites
+    assert lat.timestep() == 1
+    assert lat.class_counts().sum() == g.n_sites

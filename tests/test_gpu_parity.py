@@ -1,0 +1,173 @@
+"""Parity of the HIP path (called through the C ABI) with the CPU oracle.
+
+The bar is 1e-12 relative for FP64 distributions, density, velocity and wall
+shear stress after the same step count; the kernels evaluate every sum and
+product in the oracle's order with no fused multiply-add, so these tests
+demand bit-identical results, which implies the bar.  NaN compares equal to
+NaN (the reference's WSS is sqrt of a difference that can round below zero).
+"""
+import numpy as np
+import pytest
+
+import hemepure_amd as H
+from fixtures import (Case, IoletSpec, anisotropic_f, bifurcation_case, closed_box, cylinder_case, four_cube,
+                      sine_velocity_table)
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_parity(name, got, want, rel=1e-12):
+    got = np.asarray(got)
+    want = np.asarray(want)
+    assert got.shape == want.shape, name
+    both_nan = np.isnan(got) & np.isnan(want)
+    g = np.where(both_nan, 0.0, got)
+    w = np.where(both_nan, 0.0, want)
+    finite = np.isfinite(w) & (np.abs(w) < 1e300)
+    scale = np.abs(w[finite]).max() if finite.any() else 1.0
+    err = np.abs(g - w)
+    bound = rel * np.abs(w) + 1e-3 * rel * scale
+    bad = ~(err <= bound) & ~(g == w)
+    assert not bad.any(), f"{name}: {bad.sum()} values beyond {rel} relative, max abs err {err[bad].max()}"
+    assert np.array_equal(g, w), f"{name}: within {rel} but not bit-identical ({int((g != w).sum())} values)"
+
+
+def run_both(case: Case, steps: int, f0=None, t0=1, properties=True):
+    lat = case.product()
+    ref = case.oracle()
+    if f0 is None:
+        lat.init_equilibrium(1.0, t0)
+        ref.init_equilibrium(1.0)
+    else:
+        lat.upload_f(f0, t0)
+        ref.set_f(f0)
+    lat.step(steps, properties=properties)
+    ref.run(steps, properties=properties, threads=0)
+    return lat, ref
+
+
+def compare_all(lat, ref, tag):
+    assert_parity(f"{tag} f", lat.download_f(), ref.f())
+    rho, u, wss = lat.fields()
+    orho, ou, owss = ref.fields()
+    assert_parity(f"{tag} rho", rho, orho)
+    assert_parity(f"{tag} u", u, ou)
+    assert_parity(f"{tag} wss", wss, owss)
+
+
+def test_cylinder_lbgk_sbb_nash_2000_steps():
+    """Straight cylinder r = 10, L = 100, LBGK tau 0.8, SBB, Nash rho 1.001 / 0.999."""
+    case = cylinder_case()
+    lat, ref = run_both(case, 2000)
+    compare_all(lat, ref, "cylinder")
+    assert lat.timestep() == ref.timestep() == 2001
+    assert lat.stable() and ref.stable()
+
+
+@pytest.mark.parametrize("kernel", [H.KERNEL_LBGK, H.KERNEL_TRT, H.KERNEL_LES])
+@pytest.mark.parametrize("wall_bc", [H.WALL_SBB, H.WALL_BFL])
+def test_four_cube_corners_anisotropic(kernel, wall_bc):
+    """Every class incl. inlet/outlet-wall corners, q both sides of 1/2,
+    anisotropic start (LbTestsHelper.h:193-199 data, scaled to stay positive)."""
+    g = four_cube(8, seed=3)
+    case = Case(g, kernel=kernel, wall_bc=wall_bc, tau=0.7,
+                inlets=[IoletSpec(H.COSINE, (3.5, 3.5, 0.5), (0, 0, 1), density_mean=1.01, density_amp=0.002,
+                                  period=7.0)],
+                outlets=[IoletSpec(H.COSINE, (3.5, 3.5, 6.5), (0, 0, -1), density_mean=0.99)])
+    f0 = 0.05 + 0.01 * anisotropic_f(g.n_sites)
+    for steps in (1, 2, 7):
+        lat, ref = run_both(case, steps, f0=f0, t0=3)
+        compare_all(lat, ref, f"four_cube k{kernel} w{wall_bc} s{steps}")
+
+
+def test_ladd_parabolic_with_warmup():
+    g = H.Geometry.cylinder(8.0, 48, 24)
+    i0, o0 = g.inlets()[0], g.outlets()[0]
+    case = Case(g, kernel=H.KERNEL_LBGK, wall_bc=H.WALL_SBB, inlet_bc=H.IOLET_LADD, outlet_bc=H.IOLET_NASH,
+                tau=0.9,
+                inlets=[IoletSpec(H.PARABOLIC, i0["position"], i0["normal"], radius=8.0, max_speed=0.02,
+                                  warmup=150)],
+                outlets=[IoletSpec(H.COSINE, o0["position"], o0["normal"], density_mean=1.0)])
+    lat, ref = run_both(case, 300)
+    compare_all(lat, ref, "ladd parabolic")
+    _, u, _ = lat.fields()
+    assert u[:, 2].max() > 1e-3
+
+
+def test_les_cylinder_pulsatile():
+    case = cylinder_case(radius=8.0, length=64, box_xy=24, tau=0.52, rho_in=1.002, rho_out=0.999, amp=0.001,
+                         period=100.0, kernel=H.KERNEL_LES)
+    lat, ref = run_both(case, 300)
+    compare_all(lat, ref, "les")
+
+
+def test_bifurcation_small_trt_bfl_ladd_nash():
+    case = bifurcation_case(rp=8.0, lp=40.0, rd=6.5, ld=32.0, steps=400, peak=0.03, period=200.0)
+    lat, ref = run_both(case, 400)
+    compare_all(lat, ref, "ybif small")
+
+
+def test_bifurcation_full_size():
+    """The Y-bifurcation at its full size (parent r 24 L 200, daughters r 19 L 160)."""
+    case = bifurcation_case(steps=300)
+    lat, ref = run_both(case, 150)
+    compare_all(lat, ref, "ybif full")
+
+
+def test_upload_download_roundtrip_and_restart():
+    case = cylinder_case(radius=6.0, length=32, box_xy=16)
+    rng = np.random.default_rng(9)
+    lat = case.product()
+    f0 = 0.02 + 0.04 * rng.random((lat.n, 19))
+    lat.upload_f(f0, 1)
+    assert np.array_equal(lat.download_f(), f0)
+    # continuous 60 steps == 30 steps, checkpoint, fresh handle, 30 more steps
+    lat.step(60)
+    f_cont = lat.download_f()
+    a = case.product()
+    a.upload_f(f0, 1)
+    a.step(30)
+    mid = a.download_f()
+    b = case.product()
+    b.upload_f(mid, 31)
+    b.step(30)
+    assert np.array_equal(b.download_f(), f_cont)
+    assert b.timestep() == 61
+
+
+def test_stability_flag():
+    case = cylinder_case(radius=6.0, length=32, box_xy=16)
+    lat = case.product()
+    lat.init_equilibrium(1.0)
+    assert lat.stable()
+    f = lat.download_f()
+    f[17, 4] = -1e-3
+    lat.upload_f(f)
+    assert not lat.stable()
+
+
+def test_closed_box_mass_conservation_large():
+    """Size-independent property at ~0.9M sites: SBB walls conserve mass."""
+    g = closed_box(96, 96, 96)
+    case = Case(g, kernel=H.KERNEL_TRT, wall_bc=H.WALL_SBB, tau=0.6)
+    lat = case.product()
+    rng = np.random.default_rng(1)
+    w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+    f0 = w[None, :] * (1.0 + 0.02 * rng.uniform(-1, 1, (lat.n, 19)))
+    lat.upload_f(f0)
+    lat.step(200)
+    f = lat.download_f()
+    assert abs(f.sum() - f0.sum()) <= 1e-12 * f0.sum()
+    assert lat.stable()
+
+
+def test_long_table_chunks():
+    """Runs past one staging chunk of iolet values (1024 steps)."""
+    g = H.Geometry.cylinder(5.0, 24, 16)
+    i0, o0 = g.inlets()[0], g.outlets()[0]
+    table = sine_velocity_table(1400, 0.01, 300.0)
+    case = Case(g, kernel=H.KERNEL_TRT, wall_bc=H.WALL_BFL, inlet_bc=H.IOLET_LADD, outlet_bc=H.IOLET_NASH,
+                tau=0.7, inlets=[IoletSpec(H.FILEVEL, i0["position"], i0["normal"], radius=5.0, table=table)],
+                outlets=[IoletSpec(H.COSINE, o0["position"], o0["normal"])])
+    lat, ref = run_both(case, 1300)
+    compare_all(lat, ref, "chunks")

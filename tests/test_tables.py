@@ -1,0 +1,154 @@
+"""Site classification, ordering, neighbour tables and wall-link data of the
+product's host lattice builder, bit-for-bit against the oracle's restatement
+of LatticeData (src/geometry/LatticeData.cc:69-517), plus properties the
+tables must have whatever the geometry: the pull table inverts the push table
+on plain links, and every (site, direction) slot has exactly one writer.
+These run on the CPU: hlb_create builds the tables before touching a GPU.
+"""
+import numpy as np
+import pytest
+
+import hemepure_amd as H
+import oracle as O
+from fixtures import CX, CY, CZ, INV, Case, IoletSpec, closed_box, four_cube
+
+
+def _geoms():
+    yield "four_cube", four_cube()
+    yield "four_cube8", four_cube(8, seed=1)
+    yield "closed_box", closed_box(10, 9, 17)
+    yield "cylinder", H.Geometry.cylinder(10.0, 100, 24)
+    yield "stenosis", H.Geometry.stenosed_pipe(7.0, 60, 0.5, 7.0)
+    yield "ybif_small", H.Geometry.ybifurcation(8.0, 40.0, 6.5, 32.0, 30.0, 0.03, 2026)
+
+
+def _handle(g):
+    ins = [H.iolet_desc(H.COSINE, i["position"], i["normal"]) for i in g.inlets()]
+    outs = [H.iolet_desc(H.COSINE, o["position"], o["normal"]) for o in g.outlets()]
+    # FourCube names outlet 0 and inlet 0; a closed box has none.
+    return H.Lattice(g, H.params(), ins or [H.iolet_desc(H.COSINE, (0, 0, 0), (0, 0, 1))],
+                     outs or [H.iolet_desc(H.COSINE, (0, 0, 0), (0, 0, -1))])
+
+
+@pytest.mark.parametrize("name,geom", list(_geoms()), ids=[n for n, _ in _geoms()])
+def test_tables_match_oracle_bitwise(name, geom):
+    lat = _handle(geom)
+    ol = O.OracleLattice(geom)
+    assert lat.n == ol.n == geom.n_sites
+    np.testing.assert_array_equal(lat.class_counts(), ol.class_counts())
+    np.testing.assert_array_equal(lat.site_coords(), ol.site_coords())
+    push, pull = lat.neighbour_tables()
+    np.testing.assert_array_equal(push, ol.push_table())
+    for a, b in zip(lat.site_masks(), ol.site_masks()):
+        np.testing.assert_array_equal(a, b)
+    d, nrm = lat.wall_geometry()
+    np.testing.assert_array_equal(d, ol.wall_distances())
+    np.testing.assert_array_equal(nrm, ol.wall_normals())
+
+
+@pytest.mark.parametrize("name,geom", list(_geoms()), ids=[n for n, _ in _geoms()])
+def test_pull_inverts_push_and_one_writer(name, geom):
+    lat = _handle(geom)
+    n = lat.n
+    push, pull = lat.neighbour_tables()
+    wall, iol, _, _ = lat.site_masks()
+    writers = np.zeros((n, 19), dtype=np.int64)
+    writers[:, 0] += 1  # direction 0 streams to itself
+    k = np.arange(n)
+    for j in range(1, 19):
+        ib = INV[j]
+        bc = (((wall >> (ib - 1)) & 1) | ((iol >> (ib - 1)) & 1)).astype(bool)
+        src = pull[j - 1].astype(np.int64)
+        # boundary-supplied slots point at the site itself
+        assert (src[bc] == k[bc]).all()
+        plain = ~bc
+        # the push table of the source lands exactly here
+        assert (push[src[plain], j] == k[plain] * 19 + j).all()
+        writers[bc, j] += 1
+    # plain pushes: every non-rubbish push of a plain link hits one slot
+    for d in range(1, 19):
+        plain_src = ~((((wall >> (d - 1)) & 1) | ((iol >> (d - 1)) & 1)).astype(bool))
+        t = push[plain_src, d]
+        t = t[t < n * 19]
+        np.add.at(writers, (t // 19, t % 19), 1)
+    assert (writers == 1).all()
+
+
+def test_order_brute_force_tiny_box():
+    """Class-major order, then block order, then in-block order, z fastest
+    (LatticeData.h:375-436, VolumeTraverser.cc:56-70), recomputed by brute force."""
+    g = closed_box(3, 3, 3)
+    lat = _handle(g)
+    wall, _, _, _ = lat.site_masks()
+    coords = lat.site_coords()
+    bs = g.block_size
+    by, bz = g.blocks[1], g.blocks[2]
+
+    def key(c, w):
+        cls = 1 if w else 0
+        b = ((c[0] // bs) * by + c[1] // bs) * bz + c[2] // bs
+        s = ((c[0] % bs) * bs + c[1] % bs) * bs + c[2] % bs
+        return (cls, b, s)
+
+    keys = [key(coords[i], wall[i]) for i in range(lat.n)]
+    assert keys == sorted(keys)
+    # only the centre of a 3^3 box has no wall link
+    assert lat.class_counts()[0] == 1 and tuple(coords[0]) == (2, 2, 2)
+
+
+def _mutate(g, site, d, new_type):
+    lt = g.link_type.copy()
+    lt[site, d - 1] = new_type
+    return H.Geometry(g.blocks, g.coords, lt, g.link_iolet, g.link_dist, g.normal_avail, g.normal, g.iolets,
+                      g.block_size)
+
+
+def test_inconsistent_geometry_rejected():
+    g = H.Geometry.cylinder(6.0, 20, 16)
+    # a wall link turned plain: its neighbour is solid, nobody would write the slot
+    site = int(np.argmax((g.link_type == H.LINK_WALL).any(axis=1)))
+    d = int(np.argmax(g.link_type[site] == H.LINK_WALL)) + 1
+    with pytest.raises(H.HlbError) as e:
+        _handle(_mutate(g, site, d, H.LINK_NONE))
+    assert e.value.code == -2 and "zero" in str(e.value)
+    # a plain link turned wall while the neighbour still streams back: the
+    # site's own slot gets two writers and the neighbour's slot none
+    site = int(np.argmax((g.link_type == H.LINK_NONE).all(axis=1)))
+    with pytest.raises(H.HlbError) as e:
+        _handle(_mutate(g, site, 1, H.LINK_WALL))
+    assert e.value.code == -2 and ("two" in str(e.value) or "zero" in str(e.value))
+
+
+def test_generator_geometry_sanity():
+    g = H.Geometry.cylinder(10.0, 100, 24)
+    per_layer = np.bincount(g.coords[:, 2])
+    assert (per_layer == per_layer[0]).all() and len(per_layer) == 100
+    assert abs(per_layer[0] - np.pi * 100) < 0.05 * np.pi * 100
+    # mirror symmetry of the voxelisation about the axis
+    pts = {tuple(c) for c in g.coords[g.coords[:, 2] == 0][:, :2]}
+    assert pts == {(23 - x, y) for x, y in pts} == {(y, x) for x, y in pts}
+    # every link with a fluid neighbour is plain, every other one is wall/iolet
+    fluid = {tuple(c) for c in g.coords}
+    for i in range(0, g.n_sites, 97):
+        for d in range(1, 19):
+            nb = (g.coords[i, 0] + CX[d], g.coords[i, 1] + CY[d], g.coords[i, 2] + CZ[d])
+            assert (nb in fluid) == (g.link_type[i, d - 1] == H.LINK_NONE)
+    # iolet links only on the end layers, with distance 0.5
+    z = np.repeat(g.coords[:, 2:3], 18, axis=1)
+    il = g.link_type == H.LINK_INLET
+    assert (z[il] == 0).all() and np.allclose(g.link_dist[il], 0.5)
+    ol = g.link_type == H.LINK_OUTLET
+    assert (z[ol] == 99).all()
+    # wall distances lie in (0, 1]
+    wl = g.link_type == H.LINK_WALL
+    assert (g.link_dist[wl] > 0).all() and (g.link_dist[wl] <= 1).all()
+
+
+def test_bifurcation_symmetry_and_size():
+    g = H.Geometry.ybifurcation(24.0, 200.0, 19.0, 160.0, 30.0, 0.03, 2026)
+    assert 600_000 < g.n_sites < 900_000
+    cx = (g.blocks[0] * 8 - 1) / 2
+    pts = {tuple(c) for c in g.coords}
+    mirrored = {(int(round(2 * cx - x)), y, z) for x, y, z in pts}
+    assert len(pts ^ mirrored) < 1e-3 * len(pts)
+    assert len(g.inlets()) == 1 and len(g.outlets()) == 2
