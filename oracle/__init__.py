@@ -83,8 +83,8 @@ def lib() -> C.CDLL:
             "orc_lbm_create": (vp, [vp, C.POINTER(OrcParams), C.POINTER(OrcIolet), C.c_int,
                                     C.POINTER(OrcIolet), C.c_int, C.c_char_p, C.c_int]),
             "orc_lbm_free": (None, [vp]),
-            "orc_init_equilibrium": (None, [vp, C.c_double]),
-            "orc_set_f": (None, [vp, vp]),
+            "orc_init_equilibrium": (None, [vp, C.c_double, C.c_int64]),
+            "orc_set_f": (None, [vp, vp, C.c_int64]),
             "orc_run": (C.c_int, [vp, C.c_int64, C.c_int, C.c_int, C.c_char_p, C.c_int]),
             "orc_get_f": (None, [vp, vp]),
             "orc_get_props": (None, [vp, vp, vp, vp]),
@@ -206,12 +206,12 @@ class OracleLBM:
             lib().orc_lbm_free(self._h)
             self._h = None
 
-    def init_equilibrium(self, rho0=1.0):
-        lib().orc_init_equilibrium(self._h, rho0)
+    def init_equilibrium(self, rho0=1.0, t0=1):
+        lib().orc_init_equilibrium(self._h, rho0, t0)
 
-    def set_f(self, f):
+    def set_f(self, f, t0=1):
         f = np.ascontiguousarray(f, dtype=np.float64).reshape(self.n, Q)
-        lib().orc_set_f(self._h, _p(f))
+        lib().orc_set_f(self._h, _p(f), t0)
 
     def run(self, nsteps, properties=False, threads=1):
         err = C.create_string_buffer(512)

@@ -621,7 +621,8 @@ extern "C" orc_lbm* orc_lbm_create(orc_lattice* l, const orc_params* p, const or
 extern "C" void orc_lbm_free(orc_lbm* m) { delete m; }
 
 // EquilibriumInitialCondition::SetFs, src/lb/InitialCondition.hpp:30-43.
-extern "C" void orc_init_equilibrium(orc_lbm* m, double rho0) {
+extern "C" void orc_init_equilibrium(orc_lbm* m, double rho0, int64_t t0) {
+  m->timeStep = (uint64_t)t0;
   double feq[Q];
   Feq(m->p.sse3_order != 0, rho0, 0.0, 0.0, 0.0, feq);
   const int64_t N = m->L->N;
@@ -629,7 +630,8 @@ extern "C" void orc_init_equilibrium(orc_lbm* m, double rho0) {
     for (int d = 0; d < Q; ++d) m->fOld[k * Q + d] = m->fNew[k * Q + d] = feq[d];
 }
 
-extern "C" void orc_set_f(orc_lbm* m, const double* f) {
+extern "C" void orc_set_f(orc_lbm* m, const double* f, int64_t t0) {
+  m->timeStep = (uint64_t)t0;
   const int64_t N = m->L->N;
   for (int64_t i = 0; i < N * Q; ++i) m->fOld[i] = m->fNew[i] = f[i];
 }

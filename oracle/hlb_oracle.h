@@ -105,8 +105,10 @@ void orc_wall_normals(const orc_lattice* l, double* out);      /* n*3 */
 orc_lbm* orc_lbm_create(orc_lattice* l, const orc_params* p, const orc_iolet* inlets, int n_in,
                         const orc_iolet* outlets, int n_out, char* err, int errlen);
 void orc_lbm_free(orc_lbm* m);
-void orc_init_equilibrium(orc_lbm* m, double rho0);
-void orc_set_f(orc_lbm* m, const double* f);  /* fOld = fNew = f (AoS n*19) */
+/* Initial conditions (InitialCondition.hpp:30-50) and InitialConditionBase::SetTime
+ * (InitialCondition.cc:18-21): the 1-indexed step that runs next is t0. */
+void orc_init_equilibrium(orc_lbm* m, double rho0, int64_t t0);
+void orc_set_f(orc_lbm* m, const double* f, int64_t t0);  /* fOld = fNew = f (AoS n*19) */
 /* Advance nsteps.  props_last: compute rho/u/WSS during the last step.
  * nthreads: OpenMP threads (1 = sequential).  Returns 0, or -1 on an error
  * (message in err) such as an iolet site outside a parabolic radius. */

@@ -37,10 +37,10 @@ def run_both(case: Case, steps: int, f0=None, t0=1, properties=True):
     ref = case.oracle()
     if f0 is None:
         lat.init_equilibrium(1.0, t0)
-        ref.init_equilibrium(1.0)
+        ref.init_equilibrium(1.0, t0)
     else:
         lat.upload_f(f0, t0)
-        ref.set_f(f0)
+        ref.set_f(f0, t0)
     lat.step(steps, properties=properties)
     ref.run(steps, properties=properties, threads=0)
     return lat, ref
