@@ -30,7 +30,8 @@ std::string& last_error() {
   return msg;
 }
 
-hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, hipStream_t s);
+hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, hipStream_t bulk,
+                            hipStream_t boundary);
 hipError_t launch_gather_fp64(const StepArgs& a, double* F, hipStream_t s);
 hipError_t launch_fill_fp64(double* F, int64_t N, int64_t S, const double* vals, hipStream_t s);
 hipError_t launch_unstable_fp64(const double* F, int64_t N, int64_t S, int* flag, hipStream_t s);
@@ -91,8 +92,9 @@ struct hlb_lattice {
   std::vector<hlb::HostIolet> iolets;  // inlets then outlets
   int n_in = 0, n_out = 0;
   int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipStream_t stream = nullptr;    // bulk sites; every other operation is ordered here
+  hipStream_t stream_b = nullptr;  // boundary sites, concurrent with the bulk launch
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_b = nullptr, ev_s = nullptr;
   int64_t S = 0, NB = 0, SB = 0, NI = 0, SI = 0;
   hlb::DevBuf<double> A[2], F, G, ladd_s, rho, u, wss, io_n, io_dens, io_vmax, feq_vals;
   hlb::DevBuf<uint32_t> pull, bwall, biolet;
@@ -109,12 +111,18 @@ struct hlb_lattice {
   bool device_ready = false;
 
   ~hlb_lattice() {
+    if (stream_b) {
+      (void)hipStreamSynchronize(stream_b);
+      (void)hipStreamDestroy(stream_b);
+    }
     if (stream) {
       (void)hipStreamSynchronize(stream);
       (void)hipStreamDestroy(stream);
     }
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (ev_b) (void)hipEventDestroy(ev_b);
+    if (ev_s) (void)hipEventDestroy(ev_s);
     if (h_stage) (void)hipHostFree(h_stage);
   }
 };
@@ -136,6 +144,11 @@ StepArgs make_args(hlb_lattice* h, const double* ain, double* aout) {
   StepArgs a{};
   a.Ain = ain;
   a.Aout = aout;
+  for (int j = 0; j < Q; ++j) {
+    a.ain[j] = ain ? ain + (size_t)j * h->S : nullptr;
+    a.aout[j] = aout ? aout + (size_t)j * h->S : nullptr;
+  }
+  for (int j = 0; j < 18; ++j) a.pullp[j] = h->pull.p ? h->pull.p + (size_t)j * h->S : nullptr;
   a.pull = h->pull.p;
   a.bwall = h->bwall.p;
   a.biolet = h->biolet.p;
@@ -325,8 +338,11 @@ int ensure_device(hlb_lattice* h) {
   if (h->device_ready) return HLB_OK;
   HLB_CHECK(hipGetDevice(&h->device));
   HLB_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  HLB_CHECK(hipStreamCreateWithFlags(&h->stream_b, hipStreamNonBlocking));
   HLB_CHECK(hipEventCreate(&h->ev0));
   HLB_CHECK(hipEventCreate(&h->ev1));
+  HLB_CHECK(hipEventCreateWithFlags(&h->ev_b, hipEventDisableTiming));
+  HLB_CHECK(hipEventCreateWithFlags(&h->ev_s, hipEventDisableTiming));
   const int rc = upload_lattice(h);
   if (rc != HLB_OK) return rc;
   h->device_ready = true;
@@ -363,6 +379,18 @@ int validate_params(const hlb_params& p, const hlb_iolet_desc* in, int nin, cons
   return HLB_OK;
 }
 
+// One time step: the bulk launch on the main stream and the boundary launch
+// on the second stream run concurrently (same input, disjoint outputs); each
+// stream then waits for the other so the next step sees both.
+int launch_one_step(hlb_lattice* h, StepArgs& a, bool first, bool props) {
+  HLB_CHECK(launch_step_fp64(a, h->params.kernel, first, props, h->stream, h->stream_b));
+  HLB_CHECK(hipEventRecord(h->ev_b, h->stream_b));
+  HLB_CHECK(hipStreamWaitEvent(h->stream, h->ev_b, 0));
+  HLB_CHECK(hipEventRecord(h->ev_s, h->stream));
+  HLB_CHECK(hipStreamWaitEvent(h->stream_b, h->ev_s, 0));
+  return HLB_OK;
+}
+
 int run_steps(hlb_lattice* h, int64_t nsteps, uint32_t flags) {
   if (nsteps < 0) return fail(HLB_EINVAL, "hlb_step: negative step count");
   if (!h->f_valid) return fail(HLB_EINVAL, "hlb_step: no initial condition (call hlb_init_equilibrium or hlb_upload_f)");
@@ -389,7 +417,8 @@ int run_steps(hlb_lattice* h, int64_t nsteps, uint32_t flags) {
       StepArgs a = make_args(h, ain, aout);
       a.io_dens = h->io_dens.p + s * nio;
       a.io_vmax = h->io_vmax.p + s * nio;
-      HLB_CHECK(launch_step_fp64(a, h->params.kernel, first, props, h->stream));
+      const int rc = launch_one_step(h, a, first, props);
+      if (rc != HLB_OK) return rc;
       if (!first) h->cur ^= 1;
       h->have_post = true;
     }
@@ -559,7 +588,8 @@ extern "C" int hlb_step_timed(hlb_handle h, int64_t nsteps, uint32_t flags, doub
     StepArgs a = make_args(h, ain, aout);
     a.io_dens = h->io_dens.p + s * nio;
     a.io_vmax = h->io_vmax.p + s * nio;
-    HLB_CHECK(launch_step_fp64(a, h->params.kernel, first, props, h->stream));
+    const int rc = launch_one_step(h, a, first, props);
+    if (rc != HLB_OK) return rc;
     if (!first) h->cur ^= 1;
     h->have_post = true;
   }

@@ -61,8 +61,9 @@ int build_lattice(const hlb_geometry& g, HostLattice& L) {
   if (g.n_sites > 0 && (!g.coords || !g.link_type || !g.link_iolet || !g.link_dist || !g.normal_avail ||
                         !g.normal))
     return fail(HLB_EINVAL, "hlb_create: null geometry array");
-  if (g.n_sites >= (int64_t)std::numeric_limits<int32_t>::max())
-    return fail(HLB_EINVAL, "hlb_create: more than 2^31-1 sites per handle");
+  // Device planes are addressed with 32-bit byte offsets (8-byte elements).
+  if (g.n_sites > (int64_t(1) << 29) - 64)
+    return fail(HLB_EINVAL, "hlb_create: more than 2^29-64 sites per handle");
   for (int a = 0; a < 3; ++a) L.blocks[a] = g.blocks[a];
   L.bs = g.block_size;
   const int64_t bs = g.block_size, spb = bs * bs * bs;

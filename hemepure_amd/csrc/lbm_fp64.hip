@@ -15,10 +15,10 @@
 // neighbouring rows, which the block-structured order keeps in the same or
 // adjacent 8^3 blocks.
 //
-// Sites k < off1 (collision class 0, about 90% of a vessel) take the bulk
-// path: 18 index loads, 19 gathers, no boundary logic.  Sites of classes 1-5
-// take the boundary path; its branches are uniform per direction for most
-// wavefronts.
+// Sites k < off1 (collision class 0, about 90% of a vessel) run in their own
+// launch: 18 index loads, 19 gathers, no boundary logic, so the register
+// allocation is not sized by the boundary code.  Sites of classes 1-5 run in
+// a second launch over the contiguous range [off1, N).
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -33,74 +33,89 @@ namespace {
 
 constexpr int kBlock = 256;
 
-template <int J = 1>
-__device__ __forceinline__ void gather_bulk(const StepArgs& a, int64_t k, double* f) {
-  if constexpr (J < Q) {
-    const uint32_t src = __builtin_nontemporal_load(a.pull + (int64_t)(J - 1) * a.S + k);
-    f[J] = a.Ain[(int64_t)J * a.S + src];
-    gather_bulk<J + 1>(a, k, f);
-  }
+// Loads and stores through a uniform plane base and a 32-bit element index:
+// the byte offset stays 32-bit (planes are < 2^29 elements, checked at
+// create time), so the address is an SGPR base plus one VGPR offset instead
+// of a 64-bit VGPR pair per access.
+template <typename T>
+__device__ __forceinline__ T ldp(const T* base, uint32_t i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ T ldp_nt(const T* base, uint32_t i) {
+  return __builtin_nontemporal_load(
+      reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void stp(T* base, uint32_t i, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T)) = v;
+}
+
+__device__ __forceinline__ const double* plane(const double* p, int64_t S, int j) { return p + (int64_t)j * S; }
+__device__ __forceinline__ double* plane(double* p, int64_t S, int j) { return p + (int64_t)j * S; }
+
+// Bulk sites: all 18 index loads first, then the 19 gathers.
+__device__ __forceinline__ void gather_bulk(const StepArgs& a, uint32_t k, double* f) {
+  uint32_t src[Q];
+#pragma unroll
+  for (int j = 1; j < Q; ++j) src[j] = ldp_nt(a.pullp[j - 1], k);
+  f[0] = ldp(a.ain[0], k);
+#pragma unroll
+  for (int j = 1; j < Q; ++j) f[j] = ldp(a.ain[j], src[j]);
 }
 
 // Pre-collision values at a boundary site (classes 1-5): per direction j the
 // writer is the site's link INV[j] -- iolet delegate (value stored in G),
 // wall delegate (own post-collision values) or the neighbour (pull table).
-template <int J = 1>
-__device__ __forceinline__ void gather_boundary(const StepArgs& a, int64_t k, int64_t b, uint32_t wm,
-                                                uint32_t im, double* f) {
-  if constexpr (J < Q) {
-    constexpr int IB = INV[J];
-    constexpr uint32_t bit = 1u << (IB - 1);
-    constexpr uint32_t bitJ = 1u << (J - 1);
+// Every load is issued before any selection so a wave waits on one round of
+// independent loads (plus the gathers) rather than a chain of guarded ones:
+// boundary sites are few, so the kernel is latency- rather than bandwidth-bound.
+__device__ __forceinline__ void gather_boundary_site(const StepArgs& a, uint32_t k, double* f) {
+  const uint32_t b = k - (uint32_t)a.off1;
+  const uint32_t wm = ldp(a.bwall, b), im = ldp(a.biolet, b);
+  uint32_t src[Q];
+#pragma unroll
+  for (int j = 1; j < Q; ++j) src[j] = ldp(a.pullp[j - 1], k);
+  double own[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) own[j] = ldp(a.ain[j], k);
+  double q[Q];
+  const bool bfl = a.wall_bc == HLB_WALL_BFL;
+  if (bfl) {
+#pragma unroll
+    for (int d = 1; d < Q; ++d) q[d] = (double)ldp(a.bq + (int64_t)(d - 1) * a.SB, b);
+  }
+  double g[Q];
+  const bool iolet = k >= (uint32_t)a.off2;
+  if (iolet) {
+    const uint32_t i = k - (uint32_t)a.off2;
+#pragma unroll
+    for (int j = 1; j < Q; ++j) g[j] = ldp(plane(a.G, a.SI, j), i);
+  }
+#pragma unroll
+  for (int j = 1; j < Q; ++j) f[j] = ldp(a.ain[j], src[j]);
+  f[0] = own[0];
+#pragma unroll
+  for (int j = 1; j < Q; ++j) {
+    const int ib = INV[j];
+    const uint32_t bit = 1u << (ib - 1), bitJ = 1u << (j - 1);
     if (im & bit) {
-      f[J] = a.G[(int64_t)J * a.SI + (k - a.off2)];
+      f[j] = g[j];
     } else if (wm & bit) {
-      const double fpd = a.Ain[(int64_t)IB * a.S + k];
-      if (a.wall_bc == HLB_WALL_SBB) {
-        f[J] = fpd;
-      } else {
-        const double q = (double)a.bq[(int64_t)(IB - 1) * a.SB + b];
-        if ((wm & bitJ) || q < 0.5) {
-          f[J] = fpd;
-        } else {
-          const double fpi = a.Ain[(int64_t)J * a.S + k];
-          f[J] = (fpd + (2.0 * q - 1) * fpi) / (2.0 * q);
-        }
-      }
-    } else {
-      const uint32_t src = a.pull[(int64_t)(J - 1) * a.S + k];
-      f[J] = a.Ain[(int64_t)J * a.S + src];
+      // SimpleBounceBackDelegate.h:36-44; BFL StreamLink, :44-70.
+      if (!bfl || (wm & bitJ) || q[ib] < 0.5) f[j] = own[ib];
+      else f[j] = (own[ib] + (2.0 * q[ib] - 1) * own[j]) / (2.0 * q[ib]);
     }
-    gather_boundary<J + 1>(a, k, b, wm, im, f);
   }
-}
-
-// BFL post-step for the wall links d with q < 1/2 whose opposite link is not
-// a wall (BouzidiFirdaousLallemandDelegate.h:72-93), in direction order.
-template <int D = 1>
-__device__ __forceinline__ void bfl_post(const StepArgs& a, int64_t b, uint32_t wm, double* f) {
-  if constexpr (D < Q) {
-    constexpr int J = INV[D];
-    constexpr uint32_t bit = 1u << (D - 1);
-    constexpr uint32_t bitJ = 1u << (J - 1);
-    if (wm & bit) {
-      const double q = (double)a.bq[(int64_t)(D - 1) * a.SB + b];
-      if (!(wm & bitJ) && q < 0.5) f[J] = 2.0 * q * f[J] + (1.0 - 2.0 * q) * f[D];
+  if (bfl && wm != 0) {
+    // BFL post-step for wall links d with q < 1/2 whose opposite link is not
+    // a wall (BouzidiFirdaousLallemandDelegate.h:72-93), in direction order.
+#pragma unroll
+    for (int d = 1; d < Q; ++d) {
+      const int j = INV[d];
+      if ((wm & (1u << (d - 1))) && !(wm & (1u << (j - 1))) && q[d] < 0.5)
+        f[j] = 2.0 * q[d] * f[j] + (1.0 - 2.0 * q[d]) * f[d];
     }
-    bfl_post<D + 1>(a, b, wm, f);
-  }
-}
-
-// Pre-collision gather for site k (shared by the step and the download pass).
-__device__ __forceinline__ void gather(const StepArgs& a, int64_t k, double* f) {
-  f[0] = a.Ain[k];
-  if (k < a.off1) {
-    gather_bulk(a, k, f);
-  } else {
-    const int64_t b = k - a.off1;
-    const uint32_t wm = a.bwall[b], im = a.biolet[b];
-    gather_boundary(a, k, b, wm, im, f);
-    if (a.wall_bc == HLB_WALL_BFL && wm != 0) bfl_post(a, b, wm, f);
   }
 }
 
@@ -109,7 +124,7 @@ __device__ __forceinline__ void gather(const StepArgs& a, int64_t k, double* f) 
 // (NashZerothOrderPressureDelegate.h:36-76).  Ladd: bounce-back minus the
 // moving-wall correction (LaddIoletDelegate.h:37-86).
 template <int D = 1>
-__device__ __forceinline__ void iolet_links(const StepArgs& a, int64_t i, uint32_t im, int gi, int bc,
+__device__ __forceinline__ void iolet_links(const StepArgs& a, uint32_t i, uint32_t im, int gi, int bc,
                                             double rho, double vx, double vy, double vz,
                                             const double* fpost) {
   if constexpr (D < Q) {
@@ -125,31 +140,23 @@ __device__ __forceinline__ void iolet_links(const StepArgs& a, int64_t i, uint32
         const FeqPre p = feq_pre(rg, gx, gy, gz);
         val = feq_dir<U>(p, gx, gy, gz);
       } else {
-        const double m = a.io_vmax[gi] * a.ladd_s[(int64_t)(D - 1) * a.SI + i];
+        const double m = a.io_vmax[gi] * ldp(a.ladd_s + (int64_t)(D - 1) * a.SI, i);
         const double wx = a.io_n[3 * gi] * m * rho, wy = a.io_n[3 * gi + 1] * m * rho,
                      wz = a.io_n[3 * gi + 2] * m * rho;
         const double correction =
             2. * Wt(D) * (wx * (double)CX[D] + wy * (double)CY[D] + wz * (double)CZ[D]) / CS2;
         val = fpost[D] - correction;
       }
-      a.G[(int64_t)U * a.SI + i] = val;
+      stp(plane(a.G, a.SI, U), i, val);
     }
     iolet_links<D + 1>(a, i, im, gi, bc, rho, vx, vy, vz, fpost);
   }
 }
 
-template <int KERNEL, bool FIRST, bool PROPS>
-__global__ __launch_bounds__(kBlock) void step_fp64(StepArgs a) {
-  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (k >= a.N) return;
-  double f[Q];
-  if constexpr (FIRST) {
-#pragma unroll
-    for (int j = 0; j < Q; ++j) f[j] = a.Ain[(int64_t)j * a.S + k];
-  } else {
-    gather(a, k, f);
-  }
-
+// Collision of one site and everything that follows it: the post-collision
+// row, the iolet delegates' values for the next step, the property caches.
+template <bool BOUNDARY, int KERNEL, bool PROPS>
+__device__ __forceinline__ void collide_site(const StepArgs& a, uint32_t k, const double* f) {
   // Moments and equilibrium (Lattice.h:933-948), collision.
   double rho, mx, my, mz;
   moments(f, rho, mx, my, mz);
@@ -197,27 +204,29 @@ __global__ __launch_bounds__(kBlock) void step_fp64(StepArgs a) {
   }
 
 #pragma unroll
-  for (int j = 0; j < Q; ++j) a.Aout[(int64_t)j * a.S + k] = fpost[j];
+  for (int j = 0; j < Q; ++j) stp(a.aout[j], k, fpost[j]);
 
-  if (k >= a.off2) {
-    const int64_t b = k - a.off1, i = k - a.off2;
-    const uint32_t im = a.biolet[b];
-    const int gi = a.bgi[b];
-    const int bc = gi < a.n_in ? a.inlet_bc : a.outlet_bc;
-    iolet_links(a, i, im, gi, bc, rho, ux, uy, uz, fpost);
+  if constexpr (BOUNDARY) {
+    if (k >= (uint32_t)a.off2) {
+      const uint32_t b = k - (uint32_t)a.off1, i = k - (uint32_t)a.off2;
+      const uint32_t im = ldp(a.biolet, b);
+      const int gi = ldp(a.bgi, b);
+      const int bc = gi < a.n_in ? a.inlet_bc : a.outlet_bc;
+      iolet_links(a, i, im, gi, bc, rho, ux, uy, uz, fpost);
+    }
   }
 
   if constexpr (PROPS) {
     // BaseStreamer::UpdateMinsAndMaxes (BaseStreamer.h:88-120) and the wall
     // shear stress of Lattice.h:1122-1156.
-    a.rho[k] = rho;
-    a.u[k] = ux;
-    a.u[a.S + k] = uy;
-    a.u[2 * a.S + k] = uz;
+    stp(a.rho, k, rho);
+    stp(a.u, k, ux);
+    stp(plane(a.u, a.S, 1), k, uy);
+    stp(plane(a.u, a.S, 2), k, uz);
     double w = DBL_MAX;
-    if (k >= a.off1) {
-      const int64_t b = k - a.off1;
-      if (a.bwall[b] != 0) {
+    if constexpr (BOUNDARY) {
+      const uint32_t b = k - (uint32_t)a.off1;
+      if (ldp(a.bwall, b) != 0) {
         double g[Q];
 #pragma unroll
         for (int j = 0; j < Q; ++j) g[j] = fpost[j] - f[j];
@@ -225,7 +234,8 @@ __global__ __launch_bounds__(kBlock) void step_fp64(StepArgs a) {
         const double fac = -(tau - 0.5);
         const double p00 = s.xx * fac, p10 = s.yx * fac, p11 = s.yy * fac;
         const double p20 = s.zx * fac, p21 = s.zy * fac, p22 = s.zz * fac;
-        const double n0 = (double)a.bnrm[b], n1 = (double)a.bnrm[a.SB + b], n2 = (double)a.bnrm[2 * a.SB + b];
+        const double n0 = (double)ldp(a.bnrm, b), n1 = (double)ldp(a.bnrm + a.SB, b),
+                     n2 = (double)ldp(a.bnrm + 2 * a.SB, b);
         double sv0 = p00 * n0;
         sv0 += p10 * n1;
         sv0 += p20 * n2;
@@ -244,19 +254,41 @@ __global__ __launch_bounds__(kBlock) void step_fp64(StepArgs a) {
         w = __builtin_sqrt(sq - nn * nn);
       }
     }
-    a.wss[k] = w;
+    stp(a.wss, k, w);
   }
+}
+
+// One launch per site range: BOUNDARY = false covers the class-0 sites
+// [0, off1), BOUNDARY = true the sites [off1, N) of classes 1-5.  FIRST reads
+// the pre-collision state directly (initial condition or checkpoint).
+template <bool BOUNDARY, int KERNEL, bool FIRST, bool PROPS>
+__global__ __launch_bounds__(kBlock) void step_fp64(StepArgs a) {
+  const uint32_t first = BOUNDARY ? (uint32_t)a.off1 : 0u;
+  const uint32_t end = BOUNDARY ? (uint32_t)a.N : (uint32_t)a.off1;
+  const uint32_t k = first + blockIdx.x * kBlock + threadIdx.x;
+  if (k >= end) return;
+  double f[Q];
+  if constexpr (FIRST) {
+#pragma unroll
+    for (int j = 0; j < Q; ++j) f[j] = ldp(a.ain[j], k);
+  } else if constexpr (BOUNDARY) {
+    gather_boundary_site(a, k, f);
+  } else {
+    gather_bulk(a, k, f);
+  }
+  collide_site<BOUNDARY, KERNEL, PROPS>(a, k, f);
 }
 
 // Materialises the pre-collision f (the reference's fOld entering the next
 // step) into F, SoA.
 __global__ __launch_bounds__(kBlock) void gather_fp64(StepArgs a, double* __restrict__ F) {
-  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (k >= a.N) return;
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= (uint32_t)a.N) return;
   double f[Q];
-  gather(a, k, f);
+  if (k < (uint32_t)a.off1) gather_bulk(a, k, f);
+  else gather_boundary_site(a, k, f);
 #pragma unroll
-  for (int j = 0; j < Q; ++j) F[(int64_t)j * a.S + k] = f[j];
+  for (int j = 0; j < Q; ++j) stp(plane(F, a.S, j), k, f[j]);
 }
 
 __global__ __launch_bounds__(kBlock) void fill_planes_fp64(double* __restrict__ F, int64_t N, int64_t S,
@@ -282,9 +314,12 @@ __global__ __launch_bounds__(kBlock) void unstable_fp64(const double* __restrict
 
 unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, hipStream_t s) {
-  const dim3 grid(grid_for(a.N)), block(kBlock);
-#define HLB_LAUNCH(KN, FI, PR) hipLaunchKernelGGL((step_fp64<KN, FI, PR>), grid, block, 0, s, a)
+template <bool BOUNDARY>
+static hipError_t launch_range(const StepArgs& a, int kernel, bool first, bool props, hipStream_t s) {
+  const int64_t n = BOUNDARY ? a.N - a.off1 : a.off1;
+  if (n <= 0) return hipSuccess;
+  const dim3 grid(grid_for(n)), block(kBlock);
+#define HLB_LAUNCH(KN, FI, PR) hipLaunchKernelGGL((step_fp64<BOUNDARY, KN, FI, PR>), grid, block, 0, s, a)
 #define HLB_KSEL(KN)                                   \
   if (first) {                                         \
     if (props) HLB_LAUNCH(KN, true, true);             \
@@ -301,6 +336,15 @@ hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool prop
 #undef HLB_KSEL
 #undef HLB_LAUNCH
   return hipGetLastError();
+}
+
+hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, hipStream_t bulk,
+                            hipStream_t boundary) {
+  // The boundary launch goes first: its waves are few and latency-bound, so
+  // they should hold CUs while the bulk waves stream through HBM.
+  hipError_t e = launch_range<true>(a, kernel, first, props, boundary);
+  if (e != hipSuccess) return e;
+  return launch_range<false>(a, kernel, first, props, bulk);
 }
 
 hipError_t launch_gather_fp64(const StepArgs& a, double* F, hipStream_t s) {

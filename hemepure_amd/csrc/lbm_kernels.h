@@ -22,6 +22,11 @@
 namespace hlb {
 
 struct StepArgs {
+  // Per-plane base pointers, passed as separate scalars so each access is one
+  // SGPR base plus a 32-bit VGPR offset.
+  const double* ain[19];
+  double* aout[19];
+  const uint32_t* pullp[18];
   const double* __restrict__ Ain;   // post-collision of the previous step (or F for the first)
   double* __restrict__ Aout;
   const uint32_t* __restrict__ pull;
