@@ -3,7 +3,9 @@
 //
 // A handle plays the roles of the reference's LatticeData (tables and f
 // arrays), LBM (kernel/streamer selection and the phase sequence) and the two
-// BoundaryValues objects (iolet state), for one GPU.
+// BoundaryValues objects (iolet state), for one GPU.  Every array crossing the
+// boundary is in the reference's class-major site order; the device works in
+// natural block order (see lbm_kernels.h) and the handle maps between them.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,9 +32,8 @@ std::string& last_error() {
   return msg;
 }
 
-hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, hipStream_t bulk,
-                            hipStream_t boundary);
-hipError_t launch_gather_fp64(const StepArgs& a, double* F, hipStream_t s);
+hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, hipStream_t s);
+hipError_t launch_gather_fp64(const StepArgs& a, double* F, int64_t S, hipStream_t s);
 hipError_t launch_fill_fp64(double* F, int64_t N, int64_t S, const double* vals, hipStream_t s);
 hipError_t launch_unstable_fp64(const double* F, int64_t N, int64_t S, int* flag, hipStream_t s);
 
@@ -92,13 +93,17 @@ struct hlb_lattice {
   std::vector<hlb::HostIolet> iolets;  // inlets then outlets
   int n_in = 0, n_out = 0;
   int device = 0;
-  hipStream_t stream = nullptr;    // bulk sites; every other operation is ordered here
-  hipStream_t stream_b = nullptr;  // boundary sites, concurrent with the bulk launch
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_b = nullptr, ev_s = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // Device order (natural block order): perm[d] = reference index, inv[k] = d.
+  std::vector<int32_t> perm, inv;
+  // Boundary sites (classes 1-5) and iolet sites in device order.
+  std::vector<int32_t> bsite;  // reference index of boundary slot b
+  std::vector<int32_t> isite;  // reference index of iolet slot i
   int64_t S = 0, NB = 0, SB = 0, NI = 0, SI = 0;
   hlb::DevBuf<double> A[2], F, G, ladd_s, rho, u, wss, io_n, io_dens, io_vmax, feq_vals;
   hlb::DevBuf<uint32_t> pull, bwall, biolet;
-  hlb::DevBuf<int32_t> bgi;
+  hlb::DevBuf<int32_t> bidx, bgi, bii;
   hlb::DevBuf<float> bq, bnrm, io_nf;
   hlb::DevBuf<int> flag;
   double* h_stage = nullptr;  // pinned staging for per-step iolet values
@@ -107,22 +112,16 @@ struct hlb_lattice {
   bool f_valid = false;
   int64_t t = 1;              // 1-indexed step that runs next
   double omega = 0, half_omega = 0, half_omega_minus = 0, les_k = 0;
-  std::vector<double> h_ladd_s;  // host copy of the Ladd profile factors (18 planes of SI)
+  std::vector<double> h_ladd_s;  // Ladd profile factors, 18 planes of SI, device iolet order
   bool device_ready = false;
 
   ~hlb_lattice() {
-    if (stream_b) {
-      (void)hipStreamSynchronize(stream_b);
-      (void)hipStreamDestroy(stream_b);
-    }
     if (stream) {
       (void)hipStreamSynchronize(stream);
       (void)hipStreamDestroy(stream);
     }
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    if (ev_b) (void)hipEventDestroy(ev_b);
-    if (ev_s) (void)hipEventDestroy(ev_s);
     if (h_stage) (void)hipHostFree(h_stage);
   }
 };
@@ -142,17 +141,16 @@ int dev_fail(hipError_t e, const char* what) {
 
 StepArgs make_args(hlb_lattice* h, const double* ain, double* aout) {
   StepArgs a{};
-  a.Ain = ain;
-  a.Aout = aout;
   for (int j = 0; j < Q; ++j) {
     a.ain[j] = ain ? ain + (size_t)j * h->S : nullptr;
     a.aout[j] = aout ? aout + (size_t)j * h->S : nullptr;
   }
   for (int j = 0; j < 18; ++j) a.pullp[j] = h->pull.p ? h->pull.p + (size_t)j * h->S : nullptr;
-  a.pull = h->pull.p;
+  a.bidx = h->bidx.p;
   a.bwall = h->bwall.p;
   a.biolet = h->biolet.p;
   a.bgi = h->bgi.p;
+  a.bii = h->bii.p;
   a.bq = h->bq.p;
   a.bnrm = h->bnrm.p;
   a.G = h->G.p;
@@ -166,8 +164,6 @@ StepArgs make_args(hlb_lattice* h, const double* ain, double* aout) {
   a.io_vmax = h->io_vmax.p;
   a.N = h->L.N;
   a.S = h->S;
-  a.off1 = h->L.off[1];
-  a.off2 = h->L.off[2];
   a.SB = h->SB;
   a.SI = h->SI;
   a.n_in = h->n_in;
@@ -211,25 +207,60 @@ int iolet_values(const hlb_lattice* h, int64_t t, double* dens, double* vmax) {
   return HLB_OK;
 }
 
+int side_iolet(const hlb_lattice* h, int64_t k) {
+  const int c = h->L.collision_class(k);
+  return (c == 2 || c == 4) ? h->L.ioletId[k] : h->n_in + h->L.ioletId[k];
+}
+
+// The device order and the boundary / iolet slot lists.
+void device_order(hlb_lattice* h) {
+  const HostLattice& L = h->L;
+  const int64_t N = L.N;
+  h->perm.resize(N);
+  h->inv.resize(N);
+  const int64_t spb = (int64_t)L.bs * L.bs * L.bs;
+  int64_t d = 0;
+  for (size_t b = 0; b < L.blockSlot.size(); ++b) {
+    const int64_t slot = L.blockSlot[b];
+    if (slot < 0) continue;
+    for (int64_t s = 0; s < spb; ++s) {
+      const int32_t k = L.slotSites[slot * spb + s];
+      if (k < 0) continue;
+      h->perm[d] = k;
+      h->inv[k] = (int32_t)d;
+      ++d;
+    }
+  }
+  h->bsite.clear();
+  h->isite.clear();
+  for (int64_t dd = 0; dd < N; ++dd) {
+    const int32_t k = h->perm[dd];
+    if (k >= L.off[1]) h->bsite.push_back(k);
+    if (k >= L.off[2]) h->isite.push_back(k);
+  }
+  h->NB = (int64_t)h->bsite.size();
+  h->NI = (int64_t)h->isite.size();
+  h->S = round_up(std::max<int64_t>(N, 1), 64);
+  h->SB = round_up(std::max<int64_t>(h->NB, 1), 64);
+  h->SI = round_up(std::max<int64_t>(h->NI, 1), 64);
+}
+
 // Ladd profile factors s = 1 - r^2/a^2 at each velocity link's half-way
 // point (LaddIoletDelegate.h:66-69, InOutLetParabolicVelocity.cc:30-45).  The
 // reference stops with std::exit(16) when a half-way point lies outside the
 // iolet radius; here hlb_create fails with HLB_ERANGE.
 int ladd_factors(hlb_lattice* h) {
   const HostLattice& L = h->L;
-  h->NI = L.N - L.off[2];
-  h->SI = round_up(std::max<int64_t>(h->NI, 1), 64);
   const int64_t SI = h->SI;
   std::vector<double>& s = h->h_ladd_s;
   s.assign((size_t)18 * SI, 0.0);
   for (int64_t i = 0; i < h->NI; ++i) {
-    const int64_t k = L.off[2] + i;
+    const int64_t k = h->isite[i];
     const int c = L.collision_class(k);
     const bool inletSide = (c == 2 || c == 4);
     const int bc = inletSide ? h->params.inlet_bc : h->params.outlet_bc;
     if (bc != HLB_IOLET_LADD) continue;
-    const int g = inletSide ? L.ioletId[k] : h->n_in + L.ioletId[k];
-    const HostIolet& io = h->iolets[g];
+    const HostIolet& io = h->iolets[side_iolet(h, k)];
     for (int d = 1; d < Q; ++d) {
       if (!((L.iolet[k] >> (d - 1)) & 1u)) continue;
       const double hx = (double)L.coords[3 * k] + 0.5 * CX[d];
@@ -253,17 +284,15 @@ int ladd_factors(hlb_lattice* h) {
 
 int upload_lattice(hlb_lattice* h) {
   const HostLattice& L = h->L;
-  const int64_t N = L.N;
-  h->S = round_up(std::max<int64_t>(N, 1), 64);
-  h->NB = N - L.off[1];
-  h->SB = round_up(std::max<int64_t>(h->NB, 1), 64);
-  const int64_t S = h->S, SB = h->SB, SI = h->SI;
+  const int64_t N = L.N, S = h->S, SB = h->SB, SI = h->SI, NB = h->NB;
   HLB_CHECK(h->A[0].alloc((size_t)Q * S));
   HLB_CHECK(h->A[1].alloc((size_t)Q * S));
   HLB_CHECK(h->pull.alloc((size_t)18 * S));
+  HLB_CHECK(h->bidx.alloc((size_t)S));
   HLB_CHECK(h->bwall.alloc((size_t)SB));
   HLB_CHECK(h->biolet.alloc((size_t)SB));
   HLB_CHECK(h->bgi.alloc((size_t)SB));
+  HLB_CHECK(h->bii.alloc((size_t)SB));
   HLB_CHECK(h->bq.alloc((size_t)18 * SB));
   HLB_CHECK(h->bnrm.alloc((size_t)3 * SB));
   HLB_CHECK(h->G.alloc((size_t)Q * SI));
@@ -277,27 +306,34 @@ int upload_lattice(hlb_lattice* h) {
   HLB_CHECK(hipMemsetAsync(h->A[1].p, 0, sizeof(double) * Q * S, h->stream));
   HLB_CHECK(hipMemsetAsync(h->G.p, 0, sizeof(double) * Q * SI, h->stream));
 
-  // Pull table, padded planes.
+  // Pull table in device slots, flag on plane 0, compact boundary index.
   {
     std::vector<uint32_t> tmp((size_t)18 * S, 0);
     for (int j = 0; j < 18; ++j)
-      std::memcpy(&tmp[(size_t)j * S], &L.pull[(size_t)j * N], sizeof(uint32_t) * N);
+      for (int64_t d = 0; d < N; ++d) tmp[(size_t)j * S + d] = (uint32_t)h->inv[L.pull[(size_t)j * N + h->perm[d]]];
+    std::vector<int32_t> bidx(S, -1);
+    for (int64_t b = 0; b < NB; ++b) {
+      const int32_t d = h->inv[h->bsite[b]];
+      bidx[d] = (int32_t)b;
+      tmp[d] |= kBoundaryFlag;
+    }
     HLB_CHECK(hipMemcpy(h->pull.p, tmp.data(), sizeof(uint32_t) * tmp.size(), hipMemcpyHostToDevice));
+    HLB_CHECK(hipMemcpy(h->bidx.p, bidx.data(), sizeof(int32_t) * bidx.size(), hipMemcpyHostToDevice));
   }
-  // Boundary arrays.
+  // Boundary arrays in device order.
   {
-    const int64_t NB = h->NB, o1 = L.off[1];
+    std::vector<int32_t> iolIndex(N, -1);
+    for (int64_t i = 0; i < h->NI; ++i) iolIndex[h->isite[i]] = (int32_t)i;
     std::vector<uint32_t> w(SB, 0), io(SB, 0);
-    std::vector<int32_t> gi(SB, -1);
+    std::vector<int32_t> gi(SB, -1), ii(SB, -1);
     std::vector<float> q((size_t)18 * SB, 0.0f), nr((size_t)3 * SB, 0.0f);
     for (int64_t b = 0; b < NB; ++b) {
-      const int64_t k = o1 + b;
+      const int64_t k = h->bsite[b];
       w[b] = L.wall[k];
       io[b] = L.iolet[k];
-      if (L.iolet[k] != 0) {
-        const int c = L.collision_class(k);
-        const bool inletSide = (c == 2 || c == 4);
-        gi[b] = inletSide ? L.ioletId[k] : h->n_in + L.ioletId[k];
+      if (k >= L.off[2]) {
+        gi[b] = side_iolet(h, k);
+        ii[b] = iolIndex[k];
       }
       for (int d = 0; d < 18; ++d) q[(size_t)d * SB + b] = L.q[k * 18 + d];
       for (int a = 0; a < 3; ++a) nr[(size_t)a * SB + b] = L.nrm[k * 3 + a];
@@ -305,6 +341,7 @@ int upload_lattice(hlb_lattice* h) {
     HLB_CHECK(hipMemcpy(h->bwall.p, w.data(), sizeof(uint32_t) * SB, hipMemcpyHostToDevice));
     HLB_CHECK(hipMemcpy(h->biolet.p, io.data(), sizeof(uint32_t) * SB, hipMemcpyHostToDevice));
     HLB_CHECK(hipMemcpy(h->bgi.p, gi.data(), sizeof(int32_t) * SB, hipMemcpyHostToDevice));
+    HLB_CHECK(hipMemcpy(h->bii.p, ii.data(), sizeof(int32_t) * SB, hipMemcpyHostToDevice));
     HLB_CHECK(hipMemcpy(h->bq.p, q.data(), sizeof(float) * q.size(), hipMemcpyHostToDevice));
     HLB_CHECK(hipMemcpy(h->bnrm.p, nr.data(), sizeof(float) * nr.size(), hipMemcpyHostToDevice));
   }
@@ -338,11 +375,8 @@ int ensure_device(hlb_lattice* h) {
   if (h->device_ready) return HLB_OK;
   HLB_CHECK(hipGetDevice(&h->device));
   HLB_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-  HLB_CHECK(hipStreamCreateWithFlags(&h->stream_b, hipStreamNonBlocking));
   HLB_CHECK(hipEventCreate(&h->ev0));
   HLB_CHECK(hipEventCreate(&h->ev1));
-  HLB_CHECK(hipEventCreateWithFlags(&h->ev_b, hipEventDisableTiming));
-  HLB_CHECK(hipEventCreateWithFlags(&h->ev_s, hipEventDisableTiming));
   const int rc = upload_lattice(h);
   if (rc != HLB_OK) return rc;
   h->device_ready = true;
@@ -379,61 +413,63 @@ int validate_params(const hlb_params& p, const hlb_iolet_desc* in, int nin, cons
   return HLB_OK;
 }
 
-// One time step: the bulk launch on the main stream and the boundary launch
-// on the second stream run concurrently (same input, disjoint outputs); each
-// stream then waits for the other so the next step sees both.
-int launch_one_step(hlb_lattice* h, StepArgs& a, bool first, bool props) {
-  HLB_CHECK(launch_step_fp64(a, h->params.kernel, first, props, h->stream, h->stream_b));
-  HLB_CHECK(hipEventRecord(h->ev_b, h->stream_b));
-  HLB_CHECK(hipStreamWaitEvent(h->stream, h->ev_b, 0));
-  HLB_CHECK(hipEventRecord(h->ev_s, h->stream));
-  HLB_CHECK(hipStreamWaitEvent(h->stream_b, h->ev_s, 0));
+// Stages the per-step iolet values of steps t .. t+n-1 for the device.
+int stage_iolet_values(hlb_lattice* h, int64_t t, int64_t n) {
+  const int nio = std::max<int>(1, (int)h->iolets.size());
+  double* dens = h->h_stage;
+  double* vmax = h->h_stage + (size_t)kChunk * nio;
+  for (int64_t s = 0; s < n; ++s) {
+    const int rc = iolet_values(h, t + s, dens + s * nio, vmax + s * nio);
+    if (rc != HLB_OK) return rc;
+  }
+  HLB_CHECK(hipMemcpyAsync(h->io_dens.p, dens, sizeof(double) * n * nio, hipMemcpyHostToDevice, h->stream));
+  HLB_CHECK(hipMemcpyAsync(h->io_vmax.p, vmax, sizeof(double) * n * nio, hipMemcpyHostToDevice, h->stream));
+  return HLB_OK;
+}
+
+// Enqueues the launches of n steps whose iolet values are staged at rows 0..n-1.
+int enqueue_steps(hlb_lattice* h, int64_t n, bool props_last) {
+  const int nio = std::max<int>(1, (int)h->iolets.size());
+  for (int64_t s = 0; s < n; ++s) {
+    const bool first = !h->have_post;
+    const bool props = props_last && (s == n - 1);
+    const double* ain = first ? h->F.p : h->A[h->cur].p;
+    double* aout = first ? h->A[h->cur].p : h->A[h->cur ^ 1].p;
+    StepArgs a = make_args(h, ain, aout);
+    a.io_dens = h->io_dens.p + s * nio;
+    a.io_vmax = h->io_vmax.p + s * nio;
+    HLB_CHECK(launch_step_fp64(a, h->params.kernel, first, props, h->stream));
+    if (!first) h->cur ^= 1;
+    h->have_post = true;
+  }
+  h->t += n;
   return HLB_OK;
 }
 
 int run_steps(hlb_lattice* h, int64_t nsteps, uint32_t flags) {
   if (nsteps < 0) return fail(HLB_EINVAL, "hlb_step: negative step count");
   if (!h->f_valid) return fail(HLB_EINVAL, "hlb_step: no initial condition (call hlb_init_equilibrium or hlb_upload_f)");
-  const int nio = std::max<int>(1, (int)h->iolets.size());
   int64_t done = 0;
   while (done < nsteps) {
     const int64_t chunk = std::min<int64_t>(kChunk, nsteps - done);
-    // Stage this chunk's iolet values; the copy waits for the previous
-    // chunk's launches that read the same device buffer (stream order).
+    // The staging buffers are reused per chunk: wait for the launches that
+    // read the previous chunk's values.
     HLB_CHECK(hipStreamSynchronize(h->stream));
-    double* dens = h->h_stage;
-    double* vmax = h->h_stage + (size_t)kChunk * nio;
-    for (int64_t s = 0; s < chunk; ++s) {
-      const int rc = iolet_values(h, h->t + done + s, dens + s * nio, vmax + s * nio);
-      if (rc != HLB_OK) return rc;
-    }
-    HLB_CHECK(hipMemcpyAsync(h->io_dens.p, dens, sizeof(double) * chunk * nio, hipMemcpyHostToDevice, h->stream));
-    HLB_CHECK(hipMemcpyAsync(h->io_vmax.p, vmax, sizeof(double) * chunk * nio, hipMemcpyHostToDevice, h->stream));
-    for (int64_t s = 0; s < chunk; ++s) {
-      const bool first = !h->have_post;
-      const bool props = (flags & HLB_STEP_PROPERTIES) && (done + s == nsteps - 1);
-      const double* ain = first ? h->F.p : h->A[h->cur].p;
-      double* aout = first ? h->A[h->cur].p : h->A[h->cur ^ 1].p;
-      StepArgs a = make_args(h, ain, aout);
-      a.io_dens = h->io_dens.p + s * nio;
-      a.io_vmax = h->io_vmax.p + s * nio;
-      const int rc = launch_one_step(h, a, first, props);
-      if (rc != HLB_OK) return rc;
-      if (!first) h->cur ^= 1;
-      h->have_post = true;
-    }
+    int rc = stage_iolet_values(h, h->t, chunk);
+    if (rc != HLB_OK) return rc;
+    rc = enqueue_steps(h, chunk, (flags & HLB_STEP_PROPERTIES) && done + chunk == nsteps);
+    if (rc != HLB_OK) return rc;
     done += chunk;
   }
-  h->t += nsteps;
   return HLB_OK;
 }
 
-// Pre-collision f entering step t into F (SoA).
+// Pre-collision f entering step t into F (device order, SoA).
 int materialise_f(hlb_lattice* h) {
   if (!h->have_post) return HLB_OK;  // F already holds it
   if (!h->F.p) HLB_CHECK(h->F.alloc((size_t)Q * h->S));
   StepArgs a = make_args(h, h->A[h->cur].p, nullptr);
-  HLB_CHECK(launch_gather_fp64(a, h->F.p, h->stream));
+  HLB_CHECK(launch_gather_fp64(a, h->F.p, h->S, h->stream));
   return HLB_OK;
 }
 
@@ -509,6 +545,7 @@ extern "C" int hlb_create(const hlb_geometry* geom, const hlb_params* params, co
       k = k * C_smag;
       h->les_k = k;
     }
+    device_order(h.get());
     rc = ladd_factors(h.get());
     if (rc != HLB_OK) return rc;
     *out = h.release();
@@ -549,8 +586,10 @@ extern "C" int hlb_upload_f(hlb_handle h, const double* f, int64_t t0) {
   }
   const int64_t N = h->L.N, S = h->S;
   std::vector<double> soa((size_t)Q * S, 0.0);
-  for (int64_t k = 0; k < N; ++k)
-    for (int j = 0; j < Q; ++j) soa[(size_t)j * S + k] = f[k * Q + j];
+  for (int64_t d = 0; d < N; ++d) {
+    const double* src = f + (size_t)h->perm[d] * Q;
+    for (int j = 0; j < Q; ++j) soa[(size_t)j * S + d] = src[j];
+  }
   if (!h->F.p) HLB_CHECK(h->F.alloc((size_t)Q * S));
   HLB_CHECK(hipMemcpy(h->F.p, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
   h->have_post = false;
@@ -566,39 +605,19 @@ extern "C" int hlb_step(hlb_handle h, int64_t nsteps, uint32_t flags) {
 
 extern "C" int hlb_step_timed(hlb_handle h, int64_t nsteps, uint32_t flags, double* ms) {
   if (!h || !ms) return fail(HLB_EINVAL, "hlb_step_timed: null argument");
-  if (nsteps > kChunk) return fail(HLB_EINVAL, "hlb_step_timed: at most 1024 steps per timed call");
-  HLB_CHECK(hipStreamSynchronize(h->stream));
-  // Stage iolet values before the timed region.
-  const int nio = std::max<int>(1, (int)h->iolets.size());
-  double* dens = h->h_stage;
-  double* vmax = h->h_stage + (size_t)kChunk * nio;
-  for (int64_t s = 0; s < nsteps; ++s) {
-    const int rc = iolet_values(h, h->t + s, dens + s * nio, vmax + s * nio);
-    if (rc != HLB_OK) return rc;
-  }
-  HLB_CHECK(hipMemcpyAsync(h->io_dens.p, dens, sizeof(double) * nsteps * nio, hipMemcpyHostToDevice, h->stream));
-  HLB_CHECK(hipMemcpyAsync(h->io_vmax.p, vmax, sizeof(double) * nsteps * nio, hipMemcpyHostToDevice, h->stream));
+  if (nsteps < 0 || nsteps > kChunk) return fail(HLB_EINVAL, "hlb_step_timed: 0 to 1024 steps per timed call");
   if (!h->f_valid) return fail(HLB_EINVAL, "hlb_step_timed: no initial condition");
+  HLB_CHECK(hipStreamSynchronize(h->stream));
+  int rc = stage_iolet_values(h, h->t, nsteps);
+  if (rc != HLB_OK) return rc;
   HLB_CHECK(hipEventRecord(h->ev0, h->stream));
-  for (int64_t s = 0; s < nsteps; ++s) {
-    const bool first = !h->have_post;
-    const bool props = (flags & HLB_STEP_PROPERTIES) && (s == nsteps - 1);
-    const double* ain = first ? h->F.p : h->A[h->cur].p;
-    double* aout = first ? h->A[h->cur].p : h->A[h->cur ^ 1].p;
-    StepArgs a = make_args(h, ain, aout);
-    a.io_dens = h->io_dens.p + s * nio;
-    a.io_vmax = h->io_vmax.p + s * nio;
-    const int rc = launch_one_step(h, a, first, props);
-    if (rc != HLB_OK) return rc;
-    if (!first) h->cur ^= 1;
-    h->have_post = true;
-  }
+  rc = enqueue_steps(h, nsteps, (flags & HLB_STEP_PROPERTIES) != 0);
+  if (rc != HLB_OK) return rc;
   HLB_CHECK(hipEventRecord(h->ev1, h->stream));
   HLB_CHECK(hipEventSynchronize(h->ev1));
   float elapsed = 0.0f;
   HLB_CHECK(hipEventElapsedTime(&elapsed, h->ev0, h->ev1));
   *ms = elapsed;
-  h->t += nsteps;
   return HLB_OK;
 }
 
@@ -618,8 +637,10 @@ extern "C" int hlb_download_f(hlb_handle h, double* f) {
   std::vector<double> soa((size_t)Q * S);
   HLB_CHECK(hipMemcpyAsync(soa.data(), h->F.p, sizeof(double) * soa.size(), hipMemcpyDeviceToHost, h->stream));
   HLB_CHECK(hipStreamSynchronize(h->stream));
-  for (int64_t k = 0; k < N; ++k)
-    for (int j = 0; j < Q; ++j) f[k * Q + j] = soa[(size_t)j * S + k];
+  for (int64_t d = 0; d < N; ++d) {
+    double* dst = f + (size_t)h->perm[d] * Q;
+    for (int j = 0; j < Q; ++j) dst[j] = soa[(size_t)j * S + d];
+  }
   return HLB_OK;
 }
 
@@ -628,13 +649,19 @@ extern "C" int hlb_get_fields(hlb_handle h, double* rho, double* u, double* wss)
   if (!h->device_ready) return fail(HLB_EINVAL, "hlb_get_fields: no state");
   const int64_t N = h->L.N, S = h->S;
   HLB_CHECK(hipStreamSynchronize(h->stream));
-  if (rho) HLB_CHECK(hipMemcpy(rho, h->rho.p, sizeof(double) * N, hipMemcpyDeviceToHost));
-  if (wss) HLB_CHECK(hipMemcpy(wss, h->wss.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+  std::vector<double> tmp((size_t)3 * S);
+  if (rho) {
+    HLB_CHECK(hipMemcpy(tmp.data(), h->rho.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    for (int64_t d = 0; d < N; ++d) rho[h->perm[d]] = tmp[d];
+  }
+  if (wss) {
+    HLB_CHECK(hipMemcpy(tmp.data(), h->wss.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    for (int64_t d = 0; d < N; ++d) wss[h->perm[d]] = tmp[d];
+  }
   if (u) {
-    std::vector<double> tmp((size_t)3 * S);
     HLB_CHECK(hipMemcpy(tmp.data(), h->u.p, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
-    for (int64_t k = 0; k < N; ++k)
-      for (int a = 0; a < 3; ++a) u[3 * k + a] = tmp[(size_t)a * S + k];
+    for (int64_t d = 0; d < N; ++d)
+      for (int a = 0; a < 3; ++a) u[3 * (size_t)h->perm[d] + a] = tmp[(size_t)a * S + d];
   }
   return HLB_OK;
 }
@@ -675,10 +702,13 @@ extern "C" int hlb_neighbour_tables(hlb_handle h, int64_t* push, uint32_t* pull)
   if (pull && !h->device_ready) {
     std::memcpy(pull, h->L.pull.data(), sizeof(uint32_t) * h->L.pull.size());
   } else if (pull) {
+    // Read back the device table and express it in the reference's order.
     const int64_t N = h->L.N, S = h->S;
     std::vector<uint32_t> tmp((size_t)18 * S);
     HLB_CHECK(hipMemcpy(tmp.data(), h->pull.p, sizeof(uint32_t) * tmp.size(), hipMemcpyDeviceToHost));
-    for (int j = 0; j < 18; ++j) std::memcpy(pull + (size_t)j * N, &tmp[(size_t)j * S], sizeof(uint32_t) * N);
+    for (int j = 0; j < 18; ++j)
+      for (int64_t d = 0; d < N; ++d)
+        pull[(size_t)j * N + h->perm[d]] = (uint32_t)h->perm[tmp[(size_t)j * S + d] & kIndexMask];
   }
   return HLB_OK;
 }

@@ -1,24 +1,23 @@
 // lbm_fp64.hip -- the fused double-precision collide+stream pass for gfx950.
 //
 // One thread per site, 256 threads (four wave64s) per workgroup, sites in the
-// reference's class-major order.  A step of the reference is
+// device's natural block order.  A step of the reference is
 //   collide at every site, push fPost[d] to the neighbour (or a boundary
 //   delegate writes the site's own slot), BFL post-step, swap
-// (src/lb/lb.hpp:192-347).  Here the same step is one pull pass: site k first
-// materialises the pre-collision f it would hold in the reference's fOld by
-// gathering the previous pass's post-collision values (plain links: the
-// neighbour's value; SimpleBounceBack / BFL: the site's own values,
+// (src/lb/lb.hpp:192-347).  Here the same step is one pull pass: each site
+// first materialises the pre-collision f it would hold in the reference's
+// fOld by gathering the previous pass's post-collision values (plain links:
+// the neighbour's value; SimpleBounceBack / BFL: the site's own values,
 // src/lb/streamers/SimpleBounceBackDelegate.h:36-44,
 // BouzidiFirdaousLallemandDelegate.h:44-93; Nash / Ladd: the value its iolet
 // delegate computed last pass), then collides and stores its own
-// post-collision row.  Every store is coalesced; the 18 gathers are reads of
-// neighbouring rows, which the block-structured order keeps in the same or
-// adjacent 8^3 blocks.
+// post-collision row.  Stores are coalesced; the 18 gathers read neighbouring
+// rows of the same or an adjacent 8^3 block.
 //
-// Sites k < off1 (collision class 0, about 90% of a vessel) run in their own
-// launch: 18 index loads, 19 gathers, no boundary logic, so the register
-// allocation is not sized by the boundary code.  Sites of classes 1-5 run in
-// a second launch over the contiguous range [off1, N).
+// Workgroups are mapped XCD-contiguously: the hardware hands workgroup w to
+// XCD w mod 8, so logical block (w mod 8) * (grid / 8) + w / 8 gives each
+// XCD one contiguous eighth of the lattice, and the lines a block's
+// neighbours also read stay in that XCD's L2.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -32,11 +31,11 @@ namespace hlb {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kXcds = 8;
 
 // Loads and stores through a uniform plane base and a 32-bit element index:
 // the byte offset stays 32-bit (planes are < 2^29 elements, checked at
-// create time), so the address is an SGPR base plus one VGPR offset instead
-// of a 64-bit VGPR pair per access.
+// create time), so the address is an SGPR base plus one VGPR offset.
 template <typename T>
 __device__ __forceinline__ T ldp(const T* base, uint32_t i) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
@@ -51,76 +50,75 @@ __device__ __forceinline__ void stp(T* base, uint32_t i, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T)) = v;
 }
 
-__device__ __forceinline__ const double* plane(const double* p, int64_t S, int j) { return p + (int64_t)j * S; }
 __device__ __forceinline__ double* plane(double* p, int64_t S, int j) { return p + (int64_t)j * S; }
 
-// Bulk sites: all 18 index loads first, then the 19 gathers.
-__device__ __forceinline__ void gather_bulk(const StepArgs& a, uint32_t k, double* f) {
+__device__ __forceinline__ uint32_t site_index() {
+  const uint32_t per = gridDim.x / kXcds;
+  const uint32_t wg = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+  return wg * kBlock + threadIdx.x;
+}
+
+// Gathers along the pull table; raw1 is plane 0 with the boundary flag.
+__device__ __forceinline__ void gather_pull(const StepArgs& a, uint32_t d, uint32_t raw1, double* f) {
   uint32_t src[Q];
+  src[1] = raw1 & kIndexMask;
 #pragma unroll
-  for (int j = 1; j < Q; ++j) src[j] = ldp_nt(a.pullp[j - 1], k);
-  f[0] = ldp(a.ain[0], k);
+  for (int j = 2; j < Q; ++j) src[j] = ldp_nt(a.pullp[j - 1], d);
+  f[0] = ldp(a.ain[0], d);
 #pragma unroll
   for (int j = 1; j < Q; ++j) f[j] = ldp(a.ain[j], src[j]);
 }
 
-// Pre-collision values at a boundary site (classes 1-5): per direction j the
-// writer is the site's link INV[j] -- iolet delegate (value stored in G),
-// wall delegate (own post-collision values) or the neighbour (pull table).
-// Every load is issued before any selection so a wave waits on one round of
-// independent loads (plus the gathers) rather than a chain of guarded ones:
-// boundary sites are few, so the kernel is latency- rather than bandwidth-bound.
-__device__ __forceinline__ void gather_boundary_site(const StepArgs& a, uint32_t k, double* f) {
-  const uint32_t b = k - (uint32_t)a.off1;
+// Boundary sites: per direction j the writer of the value is the site's link
+// INV[j] -- an iolet delegate (value kept in G), a wall delegate (the site's
+// own post-collision values) or the neighbour (already gathered).  The own
+// row and the link distances are loaded before any selection so a wave waits
+// on one round of loads.
+__device__ __forceinline__ void boundary_fixup(const StepArgs& a, uint32_t d, uint32_t b, double* f) {
   const uint32_t wm = ldp(a.bwall, b), im = ldp(a.biolet, b);
-  uint32_t src[Q];
-#pragma unroll
-  for (int j = 1; j < Q; ++j) src[j] = ldp(a.pullp[j - 1], k);
   double own[Q];
 #pragma unroll
-  for (int j = 0; j < Q; ++j) own[j] = ldp(a.ain[j], k);
-  double q[Q];
+  for (int j = 1; j < Q; ++j) own[j] = ldp(a.ain[j], d);
   const bool bfl = a.wall_bc == HLB_WALL_BFL;
-  if (bfl) {
-#pragma unroll
-    for (int d = 1; d < Q; ++d) q[d] = (double)ldp(a.bq + (int64_t)(d - 1) * a.SB, b);
-  }
-  double g[Q];
-  const bool iolet = k >= (uint32_t)a.off2;
-  if (iolet) {
-    const uint32_t i = k - (uint32_t)a.off2;
-#pragma unroll
-    for (int j = 1; j < Q; ++j) g[j] = ldp(plane(a.G, a.SI, j), i);
-  }
-#pragma unroll
-  for (int j = 1; j < Q; ++j) f[j] = ldp(a.ain[j], src[j]);
-  f[0] = own[0];
-#pragma unroll
-  for (int j = 1; j < Q; ++j) {
-    const int ib = INV[j];
-    const uint32_t bit = 1u << (ib - 1), bitJ = 1u << (j - 1);
-    if (im & bit) {
-      f[j] = g[j];
-    } else if (wm & bit) {
-      // SimpleBounceBackDelegate.h:36-44; BFL StreamLink, :44-70.
-      if (!bfl || (wm & bitJ) || q[ib] < 0.5) f[j] = own[ib];
-      else f[j] = (own[ib] + (2.0 * q[ib] - 1) * own[j]) / (2.0 * q[ib]);
-    }
-  }
+  float q[Q];
   if (bfl && wm != 0) {
-    // BFL post-step for wall links d with q < 1/2 whose opposite link is not
-    // a wall (BouzidiFirdaousLallemandDelegate.h:72-93), in direction order.
 #pragma unroll
-    for (int d = 1; d < Q; ++d) {
-      const int j = INV[d];
-      if ((wm & (1u << (d - 1))) && !(wm & (1u << (j - 1))) && q[d] < 0.5)
-        f[j] = 2.0 * q[d] * f[j] + (1.0 - 2.0 * q[d]) * f[d];
+    for (int k = 1; k < Q; ++k) q[k] = ldp(a.bq + (int64_t)(k - 1) * a.SB, b);
+  }
+  if (im != 0) {
+    const uint32_t i = (uint32_t)ldp(a.bii, b);
+#pragma unroll
+    for (int j = 1; j < Q; ++j)
+      if (im & (1u << (INV[j] - 1))) f[j] = ldp(plane(a.G, a.SI, j), i);
+  }
+  if (wm != 0) {
+#pragma unroll
+    for (int j = 1; j < Q; ++j) {
+      const int ib = INV[j];
+      const uint32_t bit = 1u << (ib - 1), bitJ = 1u << (j - 1);
+      if ((wm & bit) && !(im & bit)) {
+        // SimpleBounceBackDelegate.h:36-44; BFL StreamLink, :44-70.
+        const double qd = bfl ? (double)q[ib] : 0.0;
+        if (!bfl || (wm & bitJ) || qd < 0.5) f[j] = own[ib];
+        else f[j] = (own[ib] + (2.0 * qd - 1) * own[j]) / (2.0 * qd);
+      }
+    }
+    if (bfl) {
+      // BFL post-step for wall links k with q < 1/2 whose opposite link is
+      // not a wall (BouzidiFirdaousLallemandDelegate.h:72-93), in order.
+#pragma unroll
+      for (int k = 1; k < Q; ++k) {
+        const int j = INV[k];
+        const double qk = (double)q[k];
+        if ((wm & (1u << (k - 1))) && !(wm & (1u << (j - 1))) && qk < 0.5)
+          f[j] = 2.0 * qk * f[j] + (1.0 - 2.0 * qk) * f[k];
+      }
     }
   }
 }
 
-// Iolet delegates of one site: the value each iolet link d hands to the next
-// step in direction INV[d].  Nash: f_eq of the ghost state
+// Iolet delegates of one site: the value each iolet link k hands to the next
+// step in direction INV[k].  Nash: f_eq of the ghost state
 // (NashZerothOrderPressureDelegate.h:36-76).  Ladd: bounce-back minus the
 // moving-wall correction (LaddIoletDelegate.h:37-86).
 template <int D = 1>
@@ -153,10 +151,53 @@ __device__ __forceinline__ void iolet_links(const StepArgs& a, uint32_t i, uint3
   }
 }
 
-// Collision of one site and everything that follows it: the post-collision
-// row, the iolet delegates' values for the next step, the property caches.
-template <bool BOUNDARY, int KERNEL, bool PROPS>
-__device__ __forceinline__ void collide_site(const StepArgs& a, uint32_t k, const double* f) {
+// Wall shear stress magnitude (Lattice.h:1122-1156 with the Pi tensor of
+// :1353-1383) from the non-equilibrium change fpost - f.
+__device__ __forceinline__ double wall_shear_stress(const StepArgs& a, uint32_t b, double tau, const double* f,
+                                                    const double* fpost) {
+  double g[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) g[j] = fpost[j] - f[j];
+  const Sym3 s = second_moments(g);
+  const double fac = -(tau - 0.5);
+  const double p00 = s.xx * fac, p10 = s.yx * fac, p11 = s.yy * fac;
+  const double p20 = s.zx * fac, p21 = s.zy * fac, p22 = s.zz * fac;
+  const double n0 = (double)ldp(a.bnrm, b), n1 = (double)ldp(a.bnrm + a.SB, b),
+               n2 = (double)ldp(a.bnrm + 2 * a.SB, b);
+  double sv0 = p00 * n0;
+  sv0 += p10 * n1;
+  sv0 += p20 * n2;
+  double sv1 = p10 * n0;
+  sv1 += p11 * n1;
+  sv1 += p21 * n2;
+  double sv2 = p20 * n0;
+  sv2 += p21 * n1;
+  sv2 += p22 * n2;
+  double sq = sv0 * sv0;
+  sq += sv1 * sv1;
+  sq += sv2 * sv2;
+  double nn = sv0 * n0;
+  nn += sv1 * n1;
+  nn += sv2 * n2;
+  return __builtin_sqrt(sq - nn * nn);
+}
+
+template <int KERNEL, bool FIRST, bool PROPS>
+__global__ __launch_bounds__(kBlock) void step_fp64(StepArgs a) {
+  const uint32_t d = site_index();
+  if (d >= (uint32_t)a.N) return;
+  const uint32_t raw1 = ldp_nt(a.pullp[0], d);
+  const bool isB = (raw1 & kBoundaryFlag) != 0;
+  double f[Q];
+  if constexpr (FIRST) {
+    // Initial condition or checkpoint: f is the pre-collision state itself.
+#pragma unroll
+    for (int j = 0; j < Q; ++j) f[j] = ldp(a.ain[j], d);
+  } else {
+    gather_pull(a, d, raw1, f);
+    if (isB) boundary_fixup(a, d, (uint32_t)ldp(a.bidx, d), f);
+  }
+
   // Moments and equilibrium (Lattice.h:933-948), collision.
   double rho, mx, my, mz;
   moments(f, rho, mx, my, mz);
@@ -204,91 +245,43 @@ __device__ __forceinline__ void collide_site(const StepArgs& a, uint32_t k, cons
   }
 
 #pragma unroll
-  for (int j = 0; j < Q; ++j) stp(a.aout[j], k, fpost[j]);
+  for (int j = 0; j < Q; ++j) stp(a.aout[j], d, fpost[j]);
 
-  if constexpr (BOUNDARY) {
-    if (k >= (uint32_t)a.off2) {
-      const uint32_t b = k - (uint32_t)a.off1, i = k - (uint32_t)a.off2;
+  double w = DBL_MAX;
+  if (isB) {
+    const uint32_t b = (uint32_t)ldp(a.bidx, d);
+    const int32_t ii = ldp(a.bii, b);
+    if (ii >= 0) {
       const uint32_t im = ldp(a.biolet, b);
       const int gi = ldp(a.bgi, b);
       const int bc = gi < a.n_in ? a.inlet_bc : a.outlet_bc;
-      iolet_links(a, i, im, gi, bc, rho, ux, uy, uz, fpost);
+      iolet_links(a, (uint32_t)ii, im, gi, bc, rho, ux, uy, uz, fpost);
+    }
+    if constexpr (PROPS) {
+      if (ldp(a.bwall, b) != 0) w = wall_shear_stress(a, b, tau, f, fpost);
     }
   }
-
   if constexpr (PROPS) {
-    // BaseStreamer::UpdateMinsAndMaxes (BaseStreamer.h:88-120) and the wall
-    // shear stress of Lattice.h:1122-1156.
-    stp(a.rho, k, rho);
-    stp(a.u, k, ux);
-    stp(plane(a.u, a.S, 1), k, uy);
-    stp(plane(a.u, a.S, 2), k, uz);
-    double w = DBL_MAX;
-    if constexpr (BOUNDARY) {
-      const uint32_t b = k - (uint32_t)a.off1;
-      if (ldp(a.bwall, b) != 0) {
-        double g[Q];
-#pragma unroll
-        for (int j = 0; j < Q; ++j) g[j] = fpost[j] - f[j];
-        const Sym3 s = second_moments(g);
-        const double fac = -(tau - 0.5);
-        const double p00 = s.xx * fac, p10 = s.yx * fac, p11 = s.yy * fac;
-        const double p20 = s.zx * fac, p21 = s.zy * fac, p22 = s.zz * fac;
-        const double n0 = (double)ldp(a.bnrm, b), n1 = (double)ldp(a.bnrm + a.SB, b),
-                     n2 = (double)ldp(a.bnrm + 2 * a.SB, b);
-        double sv0 = p00 * n0;
-        sv0 += p10 * n1;
-        sv0 += p20 * n2;
-        double sv1 = p10 * n0;
-        sv1 += p11 * n1;
-        sv1 += p21 * n2;
-        double sv2 = p20 * n0;
-        sv2 += p21 * n1;
-        sv2 += p22 * n2;
-        double sq = sv0 * sv0;
-        sq += sv1 * sv1;
-        sq += sv2 * sv2;
-        double nn = sv0 * n0;
-        nn += sv1 * n1;
-        nn += sv2 * n2;
-        w = __builtin_sqrt(sq - nn * nn);
-      }
-    }
-    stp(a.wss, k, w);
+    // BaseStreamer::UpdateMinsAndMaxes (BaseStreamer.h:88-120).
+    stp(a.rho, d, rho);
+    stp(a.u, d, ux);
+    stp(plane(a.u, a.S, 1), d, uy);
+    stp(plane(a.u, a.S, 2), d, uz);
+    stp(a.wss, d, w);
   }
-}
-
-// One launch per site range: BOUNDARY = false covers the class-0 sites
-// [0, off1), BOUNDARY = true the sites [off1, N) of classes 1-5.  FIRST reads
-// the pre-collision state directly (initial condition or checkpoint).
-template <bool BOUNDARY, int KERNEL, bool FIRST, bool PROPS>
-__global__ __launch_bounds__(kBlock) void step_fp64(StepArgs a) {
-  const uint32_t first = BOUNDARY ? (uint32_t)a.off1 : 0u;
-  const uint32_t end = BOUNDARY ? (uint32_t)a.N : (uint32_t)a.off1;
-  const uint32_t k = first + blockIdx.x * kBlock + threadIdx.x;
-  if (k >= end) return;
-  double f[Q];
-  if constexpr (FIRST) {
-#pragma unroll
-    for (int j = 0; j < Q; ++j) f[j] = ldp(a.ain[j], k);
-  } else if constexpr (BOUNDARY) {
-    gather_boundary_site(a, k, f);
-  } else {
-    gather_bulk(a, k, f);
-  }
-  collide_site<BOUNDARY, KERNEL, PROPS>(a, k, f);
 }
 
 // Materialises the pre-collision f (the reference's fOld entering the next
-// step) into F, SoA.
-__global__ __launch_bounds__(kBlock) void gather_fp64(StepArgs a, double* __restrict__ F) {
-  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= (uint32_t)a.N) return;
+// step) into F, device order.
+__global__ __launch_bounds__(kBlock) void gather_fp64(StepArgs a, double* __restrict__ F, int64_t S) {
+  const uint32_t d = site_index();
+  if (d >= (uint32_t)a.N) return;
+  const uint32_t raw1 = ldp(a.pullp[0], d);
   double f[Q];
-  if (k < (uint32_t)a.off1) gather_bulk(a, k, f);
-  else gather_boundary_site(a, k, f);
+  gather_pull(a, d, raw1, f);
+  if (raw1 & kBoundaryFlag) boundary_fixup(a, d, (uint32_t)ldp(a.bidx, d), f);
 #pragma unroll
-  for (int j = 0; j < Q; ++j) stp(plane(F, a.S, j), k, f[j]);
+  for (int j = 0; j < Q; ++j) stp(plane(F, S, j), d, f[j]);
 }
 
 __global__ __launch_bounds__(kBlock) void fill_planes_fp64(double* __restrict__ F, int64_t N, int64_t S,
@@ -314,12 +307,13 @@ __global__ __launch_bounds__(kBlock) void unstable_fp64(const double* __restrict
 
 unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-template <bool BOUNDARY>
-static hipError_t launch_range(const StepArgs& a, int kernel, bool first, bool props, hipStream_t s) {
-  const int64_t n = BOUNDARY ? a.N - a.off1 : a.off1;
-  if (n <= 0) return hipSuccess;
-  const dim3 grid(grid_for(n)), block(kBlock);
-#define HLB_LAUNCH(KN, FI, PR) hipLaunchKernelGGL((step_fp64<BOUNDARY, KN, FI, PR>), grid, block, 0, s, a)
+// Site grids are padded to a multiple of the XCD count for the remapping.
+static unsigned site_grid(int64_t n) { return (grid_for(n) + kXcds - 1) / kXcds * kXcds; }
+
+hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, hipStream_t s) {
+  if (a.N <= 0) return hipSuccess;
+  const dim3 grid(site_grid(a.N)), block(kBlock);
+#define HLB_LAUNCH(KN, FI, PR) hipLaunchKernelGGL((step_fp64<KN, FI, PR>), grid, block, 0, s, a)
 #define HLB_KSEL(KN)                                   \
   if (first) {                                         \
     if (props) HLB_LAUNCH(KN, true, true);             \
@@ -338,26 +332,20 @@ static hipError_t launch_range(const StepArgs& a, int kernel, bool first, bool p
   return hipGetLastError();
 }
 
-hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, hipStream_t bulk,
-                            hipStream_t boundary) {
-  // The boundary launch goes first: its waves are few and latency-bound, so
-  // they should hold CUs while the bulk waves stream through HBM.
-  hipError_t e = launch_range<true>(a, kernel, first, props, boundary);
-  if (e != hipSuccess) return e;
-  return launch_range<false>(a, kernel, first, props, bulk);
-}
-
-hipError_t launch_gather_fp64(const StepArgs& a, double* F, hipStream_t s) {
-  hipLaunchKernelGGL(gather_fp64, dim3(grid_for(a.N)), dim3(kBlock), 0, s, a, F);
+hipError_t launch_gather_fp64(const StepArgs& a, double* F, int64_t S, hipStream_t s) {
+  if (a.N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_fp64, dim3(site_grid(a.N)), dim3(kBlock), 0, s, a, F, S);
   return hipGetLastError();
 }
 
 hipError_t launch_fill_fp64(double* F, int64_t N, int64_t S, const double* vals, hipStream_t s) {
+  if (N <= 0) return hipSuccess;
   hipLaunchKernelGGL(fill_planes_fp64, dim3(grid_for(N)), dim3(kBlock), 0, s, F, N, S, vals);
   return hipGetLastError();
 }
 
 hipError_t launch_unstable_fp64(const double* F, int64_t N, int64_t S, int* flag, hipStream_t s) {
+  if (N <= 0) return hipSuccess;
   hipLaunchKernelGGL(unstable_fp64, dim3(grid_for(N)), dim3(kBlock), 0, s, F, N, S, flag);
   return hipGetLastError();
 }

@@ -1,25 +1,37 @@
 // lbm_kernels.h -- device-side data layout shared by the kernels and the
 // host launcher.
 //
-// HBM layout (one handle, N sites in the reference's contiguous order):
-//   A[2]   post-collision distributions, 19 planes of S doubles (plane j holds
-//          direction j for every site); S = N rounded up to 64.
-//   F      pre-collision staging, same shape as A (initial state, uploads and
+// Device order.  The reference's LatticeData orders sites class-major
+// (LatticeData.h:375-436); the host keeps that order for every API array.  On
+// the device the N sites are permuted into natural block order -- blocks in
+// the reference's block order, sites in a block z-fastest, all collision
+// classes together -- so that a wavefront's 64 sites are a compact 8x8 slab
+// and the boundary sites are processed next to the neighbours they exchange
+// values with.  perm[d] is the reference index of device slot d.
+//
+// HBM layout (device order, S = N rounded up to 64):
+//   A[2]   post-collision distributions, 19 planes of S doubles.
+//   F      pre-collision staging, same shape (initial state, uploads and
 //          downloads; allocated on demand).
-//   pull   18 planes of S uint32: pull[(j-1)*S + k] is the site whose
-//          post-collision value of direction j arrives at k, or k itself
-//          where a boundary condition supplies the value.
-//   boundary arrays over the sites k >= off1 (index b = k - off1, stride SB):
-//          wall/iolet link masks, global iolet index, 18 planes of float link
-//          distances, 3 planes of float wall normals.
-//   iolet arrays over the sites k >= off2 (index i = k - off2, stride SI):
-//          G, 19 planes of the values the site's iolet delegates computed for
-//          the next step, and 18 planes of Ladd profile factors.
+//   pull   18 planes of S uint32: pull[j-1][d] is the device slot whose
+//          post-collision value of direction j arrives at d, or d itself
+//          where a boundary condition supplies the value.  Bit 31 of plane 0
+//          flags the sites of collision classes 1-5 (wall or iolet links).
+//   bidx   S int32: compact boundary index of a flagged site.
+//   boundary arrays (NB flagged sites in device order, stride SB): wall and
+//          iolet link masks, global iolet index, compact iolet index, 18
+//          planes of float link distances, 3 planes of float wall normals.
+//   iolet arrays (NI sites with iolet links, device order, stride SI): G, 19
+//          planes of the values the site's iolet delegates computed for the
+//          next step, and 18 planes of Ladd profile factors.
 #pragma once
 
 #include <cstdint>
 
 namespace hlb {
+
+constexpr uint32_t kBoundaryFlag = 0x80000000u;
+constexpr uint32_t kIndexMask = 0x7fffffffu;
 
 struct StepArgs {
   // Per-plane base pointers, passed as separate scalars so each access is one
@@ -27,12 +39,11 @@ struct StepArgs {
   const double* ain[19];
   double* aout[19];
   const uint32_t* pullp[18];
-  const double* __restrict__ Ain;   // post-collision of the previous step (or F for the first)
-  double* __restrict__ Aout;
-  const uint32_t* __restrict__ pull;
+  const int32_t* __restrict__ bidx;
   const uint32_t* __restrict__ bwall;
   const uint32_t* __restrict__ biolet;
   const int32_t* __restrict__ bgi;  // global iolet index: inlets 0..n_in-1, outlets follow
+  const int32_t* __restrict__ bii;  // compact iolet index, -1 without iolet links
   const float* __restrict__ bq;
   const float* __restrict__ bnrm;
   double* __restrict__ G;
@@ -44,7 +55,7 @@ struct StepArgs {
   const double* __restrict__ io_n;    // 3 per iolet: the normal in double (Ladd)
   const double* __restrict__ io_dens; // this step's ghost density per iolet
   const double* __restrict__ io_vmax; // this step's peak speed per iolet
-  int64_t N, S, off1, off2, SB, SI;
+  int64_t N, S, SB, SI;
   int32_t n_in;
   int32_t wall_bc;
   int32_t inlet_bc;
