@@ -17,8 +17,11 @@ sample of the same workload.
 from __future__ import annotations
 
 import argparse
+import csv
 import json
 import os
+import re
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -74,6 +77,76 @@ def cpu_baseline(case, seconds_target: float = 15.0):
     }
 
 
+CAL_BYTES = 1 << 31  # 2 GiB copied per calibration launch
+
+
+def calibrate_copy() -> int:
+    """A plain streaming kernel over 2 GiB (read 2 GiB, write 2 GiB), five
+    times: the reference whose FETCH_SIZE / WRITE_SIZE readings scale the
+    counters.  An elementwise multiply, not copy_, so it is a kernel and not a
+    DMA transfer."""
+    import torch
+
+    src = torch.zeros(CAL_BYTES // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    return 0
+
+
+def _counter_means(path: Path) -> dict:
+    sums, counts = {}, {}
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            k = r["Kernel_Name"]
+            sums[k] = sums.get(k, 0.0) + float(r["Counter_Value"])
+            counts[k] = counts.get(k, 0) + 1
+    return {k: sums[k] / counts[k] for k in sums}
+
+
+def measure_traffic(workload: str) -> dict:
+    """HBM bytes per step launch from two rocprofv3 --pmc passes, each scaled
+    by the same counter's reading for the 2 GiB calibration copy.  Runs the
+    profiled programs as child processes; this process never touches the GPU."""
+    out = {}
+    tmp = Path(os.environ.get("TMPDIR", "/tmp")) / f"hlb_pmc_{os.getpid()}"
+    me = [sys.executable, "-u", str(Path(__file__).resolve())]
+    runs = {"bench": me + ["--steps", "10", "--warmup", "2", "--no-cpu-baseline", "--workload", workload],
+            "cal": me + ["--calibrate"]}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        for tag, cmd in runs.items():
+            d = tmp / f"{tag}_{counter}"
+            full = ["rocprofv3", "--pmc", counter, "-f", "csv", "-d", str(d), "-o", tag, "--"] + cmd
+            log(f"pmc pass: {' '.join(full)}")
+            subprocess.run(["timeout", "-s", "KILL", "180"] + full, check=True, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL)
+            out[(tag, counter)] = _counter_means(next(d.rglob("*counter_collection.csv")))
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        cal = out[("cal", counter)]
+        copy_kib = max(v for k, v in cal.items() if "copy" in k.lower() or "elementwise" in k.lower())
+        factor = (CAL_BYTES / 1024.0) / copy_kib
+        step = [v for k, v in out[("bench", counter)].items() if "step_fp64" in k and "true" not in k.split("<")[1]]
+        res[counter] = {"raw_kib": max(step), "factor": factor, "bytes": max(step) * 1024.0 * factor}
+    res["bytes_per_launch"] = res["FETCH_SIZE"]["bytes"] + res["WRITE_SIZE"]["bytes"]
+    return res
+
+
+def _committed_traffic(workload: str, n: int):
+    """The bytes per launch recorded in profiles/ for this workload and site
+    count by an earlier --measure-traffic run, or None."""
+    path = ROOT / "profiles" / f"step_fp64_{workload}_traffic.md"
+    if not path.exists():
+        return None
+    text = path.read_text()
+    m_sites = re.search(r"sites:\s*(\d+)", text)
+    m_bytes = re.search(r"bytes_per_launch:\s*(\d+)", text)
+    if not (m_sites and m_bytes) or int(m_sites.group(1)) != n:
+        return None
+    return int(m_bytes.group(1))
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,7 +154,16 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="bifurcation", choices=["bifurcation", "cylinder"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate", action="store_true", help="run the 2 GiB calibration copy and exit")
+    ap.add_argument("--measure-traffic", action="store_true",
+                    help="measure HBM bytes per launch with rocprofv3 counters before the timed run")
     args = ap.parse_args()
+    if args.calibrate:
+        return calibrate_copy()
+    traffic = None
+    if args.measure_traffic:
+        traffic = measure_traffic(args.workload)
+        log(f"traffic: {json.dumps(traffic)}")
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}), flush=True)
@@ -134,7 +216,8 @@ def main() -> int:
         "config": {"workload": desc, "sites": n, "storage": "fp64 SoA, 32-bit pull table",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(traffic["bytes_per_launch"]) if traffic else _committed_traffic(args.workload, n),
                      "bytes_per_site_update": BYTES_PER_SITE["f64"], "kernel": "step_fp64"},
     }
     if not args.no_cpu_baseline:
