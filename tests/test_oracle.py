@@ -248,3 +248,61 @@ def test_poiseuille_profile_in_cylinder():
     umax = grad * 10.0 ** 2 / (4 * nu * 1.0)
     assert 0.7 * umax < uz.max() < 1.2 * umax
     assert m.stable()
+
+
+def test_ladd_reduces_to_bounce_back_at_zero_speed():
+    """LaddIoletDelegate.h:37-86: with u_wall = 0 the correction vanishes and
+    the iolet links bounce back like SimpleBounceBackDelegate.h:36-44."""
+    g = four_cube(6, seed=5)
+    f0 = 0.05 + 0.01 * anisotropic_f(g.n_sites)
+    # Ladd inlet/outlet with zero speed ...
+    ladd = Case(g, inlet_bc=H.IOLET_LADD, outlet_bc=H.IOLET_LADD, tau=0.8,
+                inlets=[IoletSpec(H.PARABOLIC, (2.5, 2.5, 0.5), (0, 0, 1), radius=10.0, max_speed=0.0)],
+                outlets=[IoletSpec(H.PARABOLIC, (2.5, 2.5, 4.5), (0, 0, -1), radius=10.0, max_speed=0.0)]).oracle()
+    ladd.set_f(f0)
+    ladd.run(1)
+    # ... equals a geometry where every iolet link is a wall link (SBB).
+    lt = g.link_type.copy()
+    lt[(lt == H.LINK_INLET) | (lt == H.LINK_OUTLET)] = H.LINK_WALL
+    gw = H.Geometry(g.blocks, g.coords, lt, np.where(lt == H.LINK_WALL, -1, g.link_iolet), g.link_dist,
+                    (lt == H.LINK_WALL).any(axis=1).astype(np.uint8), np.where(g.normal_avail[:, None] > 0,
+                    g.normal, np.float32(1.0)), [], g.block_size)
+    sbb = Case(gw, tau=0.8).oracle()
+    # the two lattices order sites differently (classes): map by coordinates
+    ka = {tuple(c): i for i, c in enumerate(ladd.lat.site_coords())}
+    kb = {tuple(c): i for i, c in enumerate(sbb.lat.site_coords())}
+    fb0 = np.empty_like(f0)
+    for c, i in ka.items():
+        fb0[kb[c]] = f0[i]
+    sbb.set_f(fb0)
+    sbb.run(1)
+    fa, fb = ladd.f(), sbb.f()
+    for c, i in ka.items():
+        np.testing.assert_array_equal(fa[i], fb[kb[c]])
+
+
+def test_ladd_correction_one_link():
+    """The moving-wall term 2 w_i rho (u_w . c_i) / c_s^2 at the half-way point
+    of one inlet link, for a parabolic profile (InOutLetParabolicVelocity.cc:28-58)."""
+    g = four_cube(6, seed=5)
+    vmax, radius, tau = 0.02, 3.0, 0.8
+    pos, nrm = (2.5, 2.5, 0.5), np.array([0.0, 0.0, 1.0])
+    m = Case(g, inlet_bc=H.IOLET_LADD, outlet_bc=H.IOLET_NASH, tau=tau,
+             inlets=[IoletSpec(H.PARABOLIC, pos, tuple(nrm), radius=radius, max_speed=vmax)],
+             outlets=[IoletSpec(H.COSINE, (2.5, 2.5, 4.5), (0, 0, -1))]).oracle()
+    f0 = 0.05 + 0.01 * anisotropic_f(g.n_sites)
+    m.set_f(f0)
+    m.run(1)
+    coords = m.lat.site_coords()
+    _, iol, _, _ = m.lat.site_masks()
+    k = int(np.argmax([(c[0], c[1], c[2]) == (2, 3, 1) for c in coords]))
+    d = 6  # c = (0, 0, -1): an inlet link of every z = 1 site
+    assert (iol[k] >> (d - 1)) & 1
+    post = helper_lbgk(f0[k], tau)
+    half = coords[k] + 0.5 * C[d]
+    disp = half - np.array(pos)
+    r2 = disp @ disp - (disp @ nrm) ** 2
+    u_wall = nrm * vmax * (1.0 - r2 / radius**2)
+    rho = f0[k].sum()
+    corr = 2.0 * W[d] * rho * (u_wall @ C[d]) / (1.0 / 3.0)
+    assert abs(m.f()[k, INV[d]] - (post[d] - corr)) < 1e-12
