@@ -108,6 +108,7 @@ _SIGS = {
     "hlb_get_fields": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "hlb_timestep": (C.c_int64, [C.c_void_p]),
     "hlb_stability": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "hlb_iolet_state": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "hlb_site_count": (C.c_int64, [C.c_void_p]),
     "hlb_class_counts": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hlb_site_coords": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -382,3 +383,9 @@ class Lattice:
         s = C.c_int()
         _check(lib().hlb_stability(self._h, C.byref(s)))
         return bool(s.value)
+
+    def iolet_state(self, outlet: bool, index: int) -> np.ndarray:
+        """(density, densityNew, flowRate, flowRateOld) of one iolet."""
+        out = np.zeros(4, dtype=np.float64)
+        _check(lib().hlb_iolet_state(self._h, int(bool(outlet)), int(index), _ptr(out)))
+        return out

@@ -36,6 +36,8 @@ hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool prop
 hipError_t launch_gather_fp64(const StepArgs& a, double* F, int64_t S, hipStream_t s);
 hipError_t launch_fill_fp64(double* F, int64_t N, int64_t S, const double* vals, hipStream_t s);
 hipError_t launch_unstable_fp64(const double* F, int64_t N, int64_t S, int* flag, hipStream_t s);
+hipError_t launch_wk_begin(WkIolet* wk, int n, double* dens_row, hipStream_t s);
+hipError_t launch_wk_end(WkIolet* wk, int n, const double* contrib, const int32_t* list, hipStream_t s);
 
 namespace {
 
@@ -50,7 +52,32 @@ struct HostIolet {
   float nf[3];    // the Vector3D<float> copy Nash uses (NashZerothOrderPressureDelegate.h:57-58)
   std::vector<double> table;
   double minSimDensity = 1.0;
+  int dirs0 = 0;               // direction closest to the normal (coupling direction)
+  int64_t centre = -1;         // centre site, reference order (BoundaryValues.cc:109-128)
+  bool centreCoupled = false;
+  bool IsWK() const { return d.kind == HLB_IOLET_WK2 || d.kind == HLB_IOLET_WK3; }
 };
+
+// The lattice direction closest to the iolet normal: the same std::sort over
+// |c_k/|c_k| - n|^2 as NashZerothOrderPressureDelegate.h:95-116.
+int direction_closest_to(const double n[3]) {
+  std::array<int, Q> dirs;
+  std::array<double, Q> dist;
+  for (int k = 0; k < Q; ++k) {
+    double cx = CX[k], cy = CY[k], cz = CZ[k];
+    if (!(cx == 0.0 && cy == 0.0 && cz == 0.0)) {
+      const double mag = std::sqrt(cx * cx + cy * cy + cz * cz);
+      cx /= mag;
+      cy /= mag;
+      cz /= mag;
+    }
+    const double ex = cx - n[0], ey = cy - n[1], ez = cz - n[2];
+    dist[k] = ex * ex + ey * ey + ez * ez;
+    dirs[k] = k;
+  }
+  std::sort(dirs.begin(), dirs.end(), [&dist](int i, int j) { return dist[i] < dist[j]; });
+  return dirs[0];
+}
 
 // InOutLetCosine::GetDensity (src/lb/iolets/InOutLetCosine.cc:37-54) at the
 // 0-indexed step t0.
@@ -106,6 +133,15 @@ struct hlb_lattice {
   hlb::DevBuf<int32_t> bidx, bgi, bii;
   hlb::DevBuf<float> bq, bnrm, io_nf;
   hlb::DevBuf<int> flag;
+  // Windkessel iolets: device state, coupling bit per iolet, per-site terms,
+  // coupling sites per iolet in the reference's site order.
+  bool any_wk = false;
+  std::vector<int32_t> wk_list;
+  std::vector<hlb::WkIolet> wk_host;
+  hlb::DevBuf<hlb::WkIolet> wk;
+  hlb::DevBuf<uint32_t> wk_bit;
+  hlb::DevBuf<double> wk_contrib;
+  hlb::DevBuf<int32_t> wk_list_d;
   double* h_stage = nullptr;  // pinned staging for per-step iolet values
   int cur = 0;                // A[cur] holds the latest post-collision values
   bool have_post = false;     // false: the next step reads the pre-collision F
@@ -162,6 +198,8 @@ StepArgs make_args(hlb_lattice* h, const double* ain, double* aout) {
   a.io_n = h->io_n.p;
   a.io_dens = h->io_dens.p;
   a.io_vmax = h->io_vmax.p;
+  a.wk_bit = h->wk_bit.p;
+  a.wk_contrib = h->wk_contrib.p;
   a.N = h->L.N;
   a.S = h->S;
   a.SB = h->SB;
@@ -282,6 +320,62 @@ int ladd_factors(hlb_lattice* h) {
   return HLB_OK;
 }
 
+// Windkessel bookkeeping: coupling direction, centre site and coupling sites
+// of every Windkessel iolet.  A site couples when it has an iolet link in
+// direction INV[dirs0] (the streamers call DoPreStreamCoupling for the link
+// whose unstreamed direction is dirs0, NashZerothOrderPressureDelegate.h:46-52).
+void windkessel_setup(hlb_lattice* h) {
+  const HostLattice& L = h->L;
+  const int nio = (int)h->iolets.size();
+  h->wk_host.assign(std::max(1, nio), WkIolet{});
+  h->wk_list.clear();
+  std::vector<std::vector<int32_t>> lists(nio);
+  std::vector<int32_t> iolIndex(L.N, -1);
+  for (int64_t i = 0; i < h->NI; ++i) iolIndex[h->isite[i]] = (int32_t)i;
+  for (int g = 0; g < nio; ++g) {
+    HostIolet& io = h->iolets[g];
+    io.dirs0 = direction_closest_to(io.n);
+    io.centre = -1;
+    for (int64_t k = 0; k < L.N && io.IsWK(); ++k) {
+      bool in = true;
+      for (int a = 0; a < 3; ++a) {
+        const double c = (double)L.coords[3 * k + a];
+        in = in && (c >= io.d.position[a] - 1.0) && (c <= io.d.position[a] + 1.0);
+      }
+      if (in) {
+        io.centre = k;
+        break;
+      }
+    }
+  }
+  for (int64_t k = L.off[2]; k < L.N; ++k) {  // reference order
+    const int g = side_iolet(h, k);
+    HostIolet& io = h->iolets[g];
+    if (!io.IsWK()) continue;
+    const int link = INV[io.dirs0];
+    if (link == 0 || !((L.iolet[k] >> (link - 1)) & 1u)) continue;
+    lists[g].push_back(iolIndex[k]);
+    if (k == io.centre) io.centreCoupled = true;
+  }
+  for (int g = 0; g < nio; ++g) {
+    const HostIolet& io = h->iolets[g];
+    WkIolet& w = h->wk_host[g];
+    w.density = w.densityNew = 1.0;
+    w.flowRate = w.flowRateOld = w.flowRateNew = w.siteCount = 0.0;
+    w.area = io.d.area;
+    w.resistance = io.d.resistance;
+    w.r_c = io.d.r_c;
+    w.r_p = io.d.r_p;
+    w.capacitance = io.d.capacitance;
+    w.kind = io.d.kind;
+    w.centre_coupled = io.centreCoupled ? 1 : 0;
+    w.list_begin = (int32_t)h->wk_list.size();
+    h->wk_list.insert(h->wk_list.end(), lists[g].begin(), lists[g].end());
+    w.list_end = (int32_t)h->wk_list.size();
+    if (io.IsWK()) h->any_wk = true;
+  }
+}
+
 int upload_lattice(hlb_lattice* h) {
   const HostLattice& L = h->L;
   const int64_t N = L.N, S = h->S, SB = h->SB, SI = h->SI, NB = h->NB;
@@ -364,6 +458,19 @@ int upload_lattice(hlb_lattice* h) {
     HLB_CHECK(hipMemcpy(h->io_nf.p, nf.data(), sizeof(float) * nf.size(), hipMemcpyHostToDevice));
     HLB_CHECK(hipMemcpy(h->io_n.p, nd.data(), sizeof(double) * nd.size(), hipMemcpyHostToDevice));
     HLB_CHECK(hipHostMalloc((void**)&h->h_stage, sizeof(double) * 2 * kChunk * nio, hipHostMallocDefault));
+    std::vector<uint32_t> bits(nio, 0u);
+    for (size_t g = 0; g < h->iolets.size(); ++g)
+      if (h->iolets[g].IsWK() && h->iolets[g].dirs0 != 0) bits[g] = 1u << (INV[h->iolets[g].dirs0] - 1);
+    HLB_CHECK(h->wk_bit.alloc(nio));
+    HLB_CHECK(hipMemcpy(h->wk_bit.p, bits.data(), sizeof(uint32_t) * nio, hipMemcpyHostToDevice));
+    HLB_CHECK(h->wk.alloc(h->wk_host.size()));
+    HLB_CHECK(hipMemcpy(h->wk.p, h->wk_host.data(), sizeof(WkIolet) * h->wk_host.size(), hipMemcpyHostToDevice));
+    HLB_CHECK(h->wk_contrib.alloc((size_t)SI));
+    HLB_CHECK(hipMemsetAsync(h->wk_contrib.p, 0, sizeof(double) * SI, h->stream));
+    HLB_CHECK(h->wk_list_d.alloc(std::max<size_t>(1, h->wk_list.size())));
+    if (!h->wk_list.empty())
+      HLB_CHECK(hipMemcpy(h->wk_list_d.p, h->wk_list.data(), sizeof(int32_t) * h->wk_list.size(),
+                          hipMemcpyHostToDevice));
   }
   HLB_CHECK(hipStreamSynchronize(h->stream));
   return HLB_OK;
@@ -398,8 +505,8 @@ int validate_params(const hlb_params& p, const hlb_iolet_desc* in, int nin, cons
     for (int i = 0; i < n; ++i) {
       const int k = v[i].kind;
       if (k < HLB_IOLET_COSINE || k > HLB_IOLET_WK3) return fail(HLB_EINVAL, "hlb_create: unknown iolet kind");
-      if (k == HLB_IOLET_WK2 || k == HLB_IOLET_WK3)
-        return fail(HLB_EINVAL, "hlb_create: Windkessel iolets are not available in this build");
+      if ((k == HLB_IOLET_WK2 || k == HLB_IOLET_WK3) && bc != HLB_IOLET_NASH)
+        return fail(HLB_EINVAL, "hlb_create: a Windkessel iolet needs the Nash pressure boundary");
       const bool vel = (k == HLB_IOLET_PARABOLIC || k == HLB_IOLET_FILEVEL);
       if (bc == HLB_IOLET_LADD && !vel)
         return fail(HLB_EINVAL, "hlb_create: the Ladd boundary needs a velocity iolet (LaddIoletDelegate.h:51-53)");
@@ -438,7 +545,11 @@ int enqueue_steps(hlb_lattice* h, int64_t n, bool props_last) {
     StepArgs a = make_args(h, ain, aout);
     a.io_dens = h->io_dens.p + s * nio;
     a.io_vmax = h->io_vmax.p + s * nio;
+    if (h->any_wk)
+      HLB_CHECK(launch_wk_begin(h->wk.p, (int)h->iolets.size(), h->io_dens.p + s * nio, h->stream));
     HLB_CHECK(launch_step_fp64(a, h->params.kernel, first, props, h->stream));
+    if (h->any_wk)
+      HLB_CHECK(launch_wk_end(h->wk.p, (int)h->iolets.size(), h->wk_contrib.p, h->wk_list_d.p, h->stream));
     if (!first) h->cur ^= 1;
     h->have_post = true;
   }
@@ -545,9 +656,12 @@ extern "C" int hlb_create(const hlb_geometry* geom, const hlb_params* params, co
       k = k * C_smag;
       h->les_k = k;
     }
+    if ((int)h->iolets.size() > kMaxIolets)
+      return fail(HLB_EINVAL, "hlb_create: at most " + std::to_string(kMaxIolets) + " iolets per lattice");
     device_order(h.get());
     rc = ladd_factors(h.get());
     if (rc != HLB_OK) return rc;
+    windkessel_setup(h.get());
     *out = h.release();
     return HLB_OK;
   } catch (const std::bad_alloc&) {
@@ -732,5 +846,30 @@ extern "C" int hlb_wall_geometry(hlb_handle h, double* dist, double* normal) {
     for (size_t i = 0; i < h->L.q.size(); ++i) dist[i] = (double)h->L.q[i];
   if (normal)
     for (size_t i = 0; i < h->L.nrm.size(); ++i) normal[i] = (double)h->L.nrm[i];
+  return HLB_OK;
+}
+
+extern "C" int hlb_iolet_state(hlb_handle h, int outlet, int index, double out[4]) {
+  if (!h || !out) return fail(HLB_EINVAL, "hlb_iolet_state: null argument");
+  const int n = outlet ? h->n_out : h->n_in;
+  if (index < 0 || index >= n) return fail(HLB_EINVAL, "hlb_iolet_state: no such iolet");
+  const int g = outlet ? h->n_in + index : index;
+  const HostIolet& io = h->iolets[g];
+  if (io.IsWK() && h->device_ready) {
+    WkIolet w;
+    HLB_CHECK(hipStreamSynchronize(h->stream));
+    HLB_CHECK(hipMemcpy(&w, h->wk.p + g, sizeof(WkIolet), hipMemcpyDeviceToHost));
+    out[0] = w.density;
+    out[1] = w.densityNew;
+    out[2] = w.flowRate;
+    out[3] = w.flowRateOld;
+    return HLB_OK;
+  }
+  std::vector<double> dens(h->iolets.size()), vmax(h->iolets.size());
+  const int rc = iolet_values(h, std::max<int64_t>(1, h->t - 1), dens.data(), vmax.data());
+  if (rc != HLB_OK) return rc;
+  out[0] = io.IsWK() ? 1.0 : dens[g];
+  out[1] = out[0];
+  out[2] = out[3] = 0.0;
   return HLB_OK;
 }

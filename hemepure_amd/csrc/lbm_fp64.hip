@@ -256,6 +256,11 @@ __global__ __launch_bounds__(kBlock) void step_fp64(StepArgs a) {
       const int gi = ldp(a.bgi, b);
       const int bc = gi < a.n_in ? a.inlet_bc : a.outlet_bc;
       iolet_links(a, (uint32_t)ii, im, gi, bc, rho, ux, uy, uz, fpost);
+      // InOutLetWK*::DoPreStreamCoupling: the coupling site's velocity.Dot(-normal).
+      if (im & a.wk_bit[gi]) {
+        const double c = ux * -a.io_n[3 * gi] + uy * -a.io_n[3 * gi + 1] + uz * -a.io_n[3 * gi + 2];
+        stp(a.wk_contrib, (uint32_t)ii, c);
+      }
     }
     if constexpr (PROPS) {
       if (ldp(a.bwall, b) != 0) w = wall_shear_stress(a, b, tau, f, fpost);
@@ -303,7 +308,67 @@ __global__ __launch_bounds__(kBlock) void unstable_fp64(const double* __restrict
   if (bad) *flag = 1;
 }
 
+// Start of a step for the Windkessel iolets: InOutLetWK*::DoComms
+// (InOutLetWK3.cc:33-52) -- density <- densityNew, flowRate <- the previous
+// step's sum scaled by area / siteCount -- and the ghost density the step's
+// Nash links read.
+__global__ void wk_begin(WkIolet* __restrict__ wk, int n, double* __restrict__ dens_row) {
+  const int g = threadIdx.x;
+  if (g >= n) return;
+  WkIolet& w = wk[g];
+  if (w.kind != HLB_IOLET_WK2 && w.kind != HLB_IOLET_WK3) return;
+  w.density = w.densityNew;
+  w.flowRate = w.flowRateNew;
+  if (w.siteCount != 0) w.flowRate = w.flowRate * w.area / w.siteCount;
+  w.flowRateNew = 0.0;
+  w.siteCount = 0;
+  dens_row[g] = w.density;
+}
+
+// End of a step: the coupling sites' terms summed in the reference's site
+// order, the centre site's pressure update (InOutLetWK2.cc:82-95,
+// InOutLetWK3.cc:70-95) and the post-stream commit (:97-106).
+__global__ void wk_end(WkIolet* __restrict__ wk, int n, const double* __restrict__ contrib,
+                       const int32_t* __restrict__ list) {
+  const int g = threadIdx.x;
+  if (g >= n) return;
+  WkIolet& w = wk[g];
+  if (w.kind != HLB_IOLET_WK2 && w.kind != HLB_IOLET_WK3) return;
+  if (w.centre_coupled) {
+    const double pressure = w.density * CS2;
+    if (w.kind == HLB_IOLET_WK2) {
+      const double R0 = w.resistance, C0 = w.capacitance;
+      const double pressureNew = R0 / (1.0 + R0 * C0) * (w.flowRate + C0 * pressure);
+      w.densityNew = pressureNew / CS2;
+    } else {
+      const double Rc = w.r_c, Rp = w.r_p, Cp = w.capacitance;
+      const double pressureNew = (Rp * Cp * pressure + (Rc + Rp) * (2.0 * w.flowRate - w.flowRateOld) +
+                                  Rc * Rp * Cp * (w.flowRate - w.flowRateOld)) /
+                                 (1.0 + Rp * Cp);
+      w.densityNew = pressureNew / CS2;
+    }
+  }
+  double sum = w.flowRateNew;
+  for (int32_t e = w.list_begin; e < w.list_end; ++e) sum += contrib[list[e]];
+  w.flowRateNew = sum;
+  w.siteCount += (double)(w.list_end - w.list_begin);
+  if (w.centre_coupled) {
+    w.density = w.densityNew;
+    w.flowRateOld = w.flowRate;
+  }
+}
+
 }  // namespace
+
+hipError_t launch_wk_begin(WkIolet* wk, int n, double* dens_row, hipStream_t s) {
+  hipLaunchKernelGGL(wk_begin, dim3(1), dim3(kMaxIolets), 0, s, wk, n, dens_row);
+  return hipGetLastError();
+}
+
+hipError_t launch_wk_end(WkIolet* wk, int n, const double* contrib, const int32_t* list, hipStream_t s) {
+  hipLaunchKernelGGL(wk_end, dim3(1), dim3(kMaxIolets), 0, s, wk, n, contrib, list);
+  return hipGetLastError();
+}
 
 unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 

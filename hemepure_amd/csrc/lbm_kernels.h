@@ -32,6 +32,7 @@ namespace hlb {
 
 constexpr uint32_t kBoundaryFlag = 0x80000000u;
 constexpr uint32_t kIndexMask = 0x7fffffffu;
+constexpr int kMaxIolets = 256;  // one thread per iolet in the Windkessel kernels
 
 struct StepArgs {
   // Per-plane base pointers, passed as separate scalars so each access is one
@@ -55,6 +56,8 @@ struct StepArgs {
   const double* __restrict__ io_n;    // 3 per iolet: the normal in double (Ladd)
   const double* __restrict__ io_dens; // this step's ghost density per iolet
   const double* __restrict__ io_vmax; // this step's peak speed per iolet
+  const uint32_t* __restrict__ wk_bit; // per iolet: the link bit of its Windkessel coupling direction
+  double* __restrict__ wk_contrib;     // per iolet site: u.(-n) of this step (coupling sites only)
   int64_t N, S, SB, SI;
   int32_t n_in;
   int32_t wall_bc;
@@ -65,6 +68,16 @@ struct StepArgs {
   double half_omega;  // 0.5*omega (TRT)
   double half_omega_minus;
   double les_k;       // 18 sqrt(2) C_s^2 evaluated in the reference's order
+};
+
+// Windkessel state of one iolet (InOutLetWK2.cc:13-18, InOutLetWK3.cc:13-18)
+// plus its constants; updated on the device between steps.
+struct WkIolet {
+  double density, densityNew, flowRate, flowRateOld, flowRateNew, siteCount;
+  double area, resistance, r_c, r_p, capacitance;
+  int32_t kind;            // HLB_IOLET_WK2 / HLB_IOLET_WK3, anything else: not a Windkessel
+  int32_t centre_coupled;  // the centre site is one of this iolet's coupling sites
+  int32_t list_begin, list_end;  // coupling sites in the reference's site order
 };
 
 }  // namespace hlb

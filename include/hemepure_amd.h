@@ -178,6 +178,13 @@ int64_t hlb_timestep(hlb_handle h);
  * distribution entering the next step is > 0 (and not NaN). */
 int hlb_stability(hlb_handle h, int* stable);
 
+/* Iolet state (outlet = 0: inlet `index`, 1: outlet `index`):
+ * out[0] density used by the last step's ghost sites (for Windkessel iolets,
+ * the committed density), out[1] densityNew, out[2] flowRate, out[3]
+ * flowRateOld -- the members of InOutLetWK3 (src/lb/iolets/InOutLetWK3.h)
+ * read by BoundaryValues::GetBoundaryDensity (BoundaryValues.cc:211-214). */
+int hlb_iolet_state(hlb_handle h, int outlet, int index, double out[4]);
+
 /* ---- lattice tables (LatticeData accessors, for parity and extraction) -- */
 int64_t hlb_site_count(hlb_handle h);          /* GetLocalFluidSiteCount */
 /* Get{MidDomain,DomainEdge}CollisionCount (LatticeData.h:375-436). */

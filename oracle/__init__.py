@@ -237,6 +237,12 @@ class OracleLBM:
     def stable(self):
         return bool(lib().orc_stable(self._h))
 
+    def iolet_state(self, outlet, index):
+        """(density, densityNew, flowRate, flowRateOld) of one iolet."""
+        out = np.zeros(4)
+        lib().orc_iolet_state(self._h, 1 if outlet else 0, int(index), _p(out))
+        return out
+
 
 def moments(f, sse3=True):
     f = np.ascontiguousarray(f, dtype=np.float64)

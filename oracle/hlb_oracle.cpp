@@ -839,8 +839,10 @@ extern "C" void orc_get_props(const orc_lbm* m, double* rho, double* u, double* 
 extern "C" int64_t orc_timestep(const orc_lbm* m) { return (int64_t)m->timeStep; }
 extern "C" void orc_iolet_state(const orc_lbm* m, int outlet, int i, double out[4]) {
   const Iolet& io = outlet ? m->outlets[i] : m->inlets[i];
+  // Only the Windkessel iolets carry densityNew; the others report their
+  // density in both slots.
   out[0] = io.IsWK() ? io.density : BoundaryDensity(io, m->timeStep - 2);
-  out[1] = io.densityNew;
+  out[1] = io.IsWK() ? io.densityNew : out[0];
   out[2] = io.flowRate;
   out[3] = io.flowRateOld;
 }

@@ -111,11 +111,18 @@ class IoletSpec:
     warmup: int = 0
     max_speed: float = 0.0
     table: np.ndarray | None = None
+    area: float = 1.0
+    resistance: float = 1.0
+    r_c: float = 1.0
+    r_p: float = 1.0
+    capacitance: float = 1.0
 
     def kw(self):
         return dict(position=self.position, normal=self.normal, radius=self.radius,
                     density_mean=self.density_mean, density_amp=self.density_amp, phase=self.phase,
-                    period=self.period, warmup=self.warmup, max_speed=self.max_speed, table=self.table)
+                    period=self.period, warmup=self.warmup, max_speed=self.max_speed, table=self.table,
+                    area=self.area, resistance=self.resistance, r_c=self.r_c, r_p=self.r_p,
+                    capacitance=self.capacitance)
 
 
 @dataclass
@@ -156,6 +163,20 @@ def cylinder_case(radius=10.0, length=100, box_xy=24, tau=0.8, rho_in=1.001, rho
                 inlets=[IoletSpec(H.COSINE, i0["position"], i0["normal"], density_mean=rho_in, density_amp=amp,
                                   period=period)],
                 outlets=[IoletSpec(H.COSINE, o0["position"], o0["normal"], density_mean=rho_out, period=period)])
+
+
+def windkessel_cylinder_case(kind=H.WK3, radius=6.0, length=40, box_xy=16, tau=0.8, rho_in=1.01,
+                             r_c=0.001, r_p=0.1, resistance=0.1, capacitance=3000.0,
+                             kernel=H.KERNEL_LBGK, wall_bc=H.WALL_SBB) -> Case:
+    """A straight cylinder driven by a constant-density Nash inlet into a
+    Windkessel outlet (InOutLetWK2 / InOutLetWK3 with the Nash delegate)."""
+    g = H.Geometry.cylinder(radius, length, box_xy)
+    i0, o0 = g.inlets()[0], g.outlets()[0]
+    area = math.pi * radius * radius
+    return Case(g, kernel=kernel, wall_bc=wall_bc, tau=tau,
+                inlets=[IoletSpec(H.COSINE, i0["position"], i0["normal"], density_mean=rho_in)],
+                outlets=[IoletSpec(kind, o0["position"], o0["normal"], radius=radius, area=area,
+                                   resistance=resistance, r_c=r_c, r_p=r_p, capacitance=capacitance)])
 
 
 def sine_velocity_table(nsteps: int, peak: float, period: float) -> np.ndarray:

@@ -11,7 +11,7 @@ import pytest
 
 import hemepure_amd as H
 from fixtures import (Case, IoletSpec, anisotropic_f, bifurcation_case, closed_box, cylinder_case, four_cube,
-                      sine_velocity_table)
+                      sine_velocity_table, windkessel_cylinder_case)
 
 pytestmark = pytest.mark.gpu
 
@@ -112,6 +112,46 @@ def test_bifurcation_full_size():
     case = bifurcation_case(steps=300)
     lat, ref = run_both(case, 150)
     compare_all(lat, ref, "ybif full")
+
+
+def compare_iolets(lat, ref, case, tag):
+    for outlet, specs in ((False, case.inlets), (True, case.outlets)):
+        for i in range(len(specs)):
+            assert_parity(f"{tag} iolet {int(outlet)}/{i}", lat.iolet_state(outlet, i), ref.iolet_state(outlet, i))
+
+
+@pytest.mark.parametrize("kind,kernel,wall_bc", [(H.WK2, H.KERNEL_LBGK, H.WALL_SBB),
+                                                 (H.WK3, H.KERNEL_TRT, H.WALL_BFL)])
+def test_windkessel_outlet_cylinder(kind, kernel, wall_bc):
+    """Nash inlet into a Windkessel outlet; state, flow-rate sums and the
+    ghost density must follow the oracle bitwise, across the 1024-step
+    staging chunks and a split into separate calls."""
+    case = windkessel_cylinder_case(kind=kind, length=24, kernel=kernel, wall_bc=wall_bc)
+    lat, ref = run_both(case, 1, properties=False)
+    compare_iolets(lat, ref, case, "wk after 1")
+    lat.step(1100, properties=True)
+    ref.run(1100, properties=True, threads=0)
+    compare_all(lat, ref, "wk")
+    compare_iolets(lat, ref, case, "wk")
+    st = lat.iolet_state(True, 0)
+    assert abs(st[0] - 1.0) > 1e-3 and st[2] > 0.1
+
+
+def test_windkessel_bifurcation_two_outlets():
+    """Two Windkessel outlets with different constants on the small Y-bifurcation."""
+    base = bifurcation_case(rp=8.0, lp=40.0, rd=6.5, ld=32.0, steps=400, peak=0.03, period=200.0)
+    outs = []
+    for j, o in enumerate(base.geom.outlets()):
+        r = o["radius"]
+        outs.append(IoletSpec(H.WK3 if j == 0 else H.WK2, o["position"], o["normal"], radius=r,
+                              area=np.pi * r * r, r_c=0.001, r_p=0.2, resistance=0.15,
+                              capacitance=2000.0 + 1000.0 * j))
+    case = Case(base.geom, kernel=base.kernel, wall_bc=base.wall_bc, inlet_bc=base.inlet_bc,
+                outlet_bc=H.IOLET_NASH, tau=base.tau, trt_lambda=base.trt_lambda, inlets=base.inlets,
+                outlets=outs)
+    lat, ref = run_both(case, 400)
+    compare_all(lat, ref, "wk ybif")
+    compare_iolets(lat, ref, case, "wk ybif")
 
 
 def test_upload_download_roundtrip_and_restart():
