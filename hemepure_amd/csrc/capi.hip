@@ -38,6 +38,11 @@ hipError_t launch_fill_fp64(double* F, int64_t N, int64_t S, const double* vals,
 hipError_t launch_unstable_fp64(const double* F, int64_t N, int64_t S, int* flag, hipStream_t s);
 hipError_t launch_wk_begin(WkIolet* wk, int n, double* dens_row, hipStream_t s);
 hipError_t launch_wk_end(WkIolet* wk, int n, const double* contrib, const int32_t* list, hipStream_t s);
+hipError_t launch_step_fp32(const Member32* members, const int32_t* wg_member, uint32_t grid, int step,
+                            hipStream_t s);
+hipError_t launch_gather_fp32(const Member32* members, int64_t N, float* F, hipStream_t s);
+hipError_t launch_fill_fp32(float* F, int64_t N, int64_t S, const float* vals, hipStream_t s);
+hipError_t launch_unstable_fp32(const float* F, int64_t N, int64_t S, int* flag, hipStream_t s);
 
 namespace {
 
@@ -133,6 +138,11 @@ struct hlb_lattice {
   hlb::DevBuf<int32_t> bidx, bgi, bii;
   hlb::DevBuf<float> bq, bnrm, io_nf;
   hlb::DevBuf<int> flag;
+  // FP32-shifted storage (params.precision == HLB_FP32_SHIFTED): float A, F,
+  // G and the one-member table the member-table kernel reads.
+  hlb::DevBuf<float> A32[2], F32, G32, feq32;
+  hlb::DevBuf<hlb::Member32> mem;
+  hlb::Member32* h_mem = nullptr;  // pinned staging of the member record
   // Windkessel iolets: device state, coupling bit per iolet, per-site terms,
   // coupling sites per iolet in the reference's site order.
   bool any_wk = false;
@@ -159,7 +169,9 @@ struct hlb_lattice {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (h_stage) (void)hipHostFree(h_stage);
+    if (h_mem) (void)hipHostFree(h_mem);
   }
+  bool fp32() const { return params.precision == HLB_FP32_SHIFTED; }
 };
 
 namespace hlb {
@@ -379,8 +391,24 @@ void windkessel_setup(hlb_lattice* h) {
 int upload_lattice(hlb_lattice* h) {
   const HostLattice& L = h->L;
   const int64_t N = L.N, S = h->S, SB = h->SB, SI = h->SI, NB = h->NB;
-  HLB_CHECK(h->A[0].alloc((size_t)Q * S));
-  HLB_CHECK(h->A[1].alloc((size_t)Q * S));
+  if (h->fp32()) {
+    HLB_CHECK(h->A32[0].alloc((size_t)Q * S));
+    HLB_CHECK(h->A32[1].alloc((size_t)Q * S));
+    HLB_CHECK(h->G32.alloc((size_t)Q * SI));
+    HLB_CHECK(h->feq32.alloc(Q));
+    HLB_CHECK(h->mem.alloc(1));
+    HLB_CHECK(hipHostMalloc((void**)&h->h_mem, sizeof(Member32), hipHostMallocDefault));
+    HLB_CHECK(hipMemsetAsync(h->A32[0].p, 0, sizeof(float) * Q * S, h->stream));
+    HLB_CHECK(hipMemsetAsync(h->A32[1].p, 0, sizeof(float) * Q * S, h->stream));
+    HLB_CHECK(hipMemsetAsync(h->G32.p, 0, sizeof(float) * Q * SI, h->stream));
+  } else {
+    HLB_CHECK(h->A[0].alloc((size_t)Q * S));
+    HLB_CHECK(h->A[1].alloc((size_t)Q * S));
+    HLB_CHECK(h->G.alloc((size_t)Q * SI));
+    HLB_CHECK(hipMemsetAsync(h->A[0].p, 0, sizeof(double) * Q * S, h->stream));
+    HLB_CHECK(hipMemsetAsync(h->A[1].p, 0, sizeof(double) * Q * S, h->stream));
+    HLB_CHECK(hipMemsetAsync(h->G.p, 0, sizeof(double) * Q * SI, h->stream));
+  }
   HLB_CHECK(h->pull.alloc((size_t)18 * S));
   HLB_CHECK(h->bidx.alloc((size_t)S));
   HLB_CHECK(h->bwall.alloc((size_t)SB));
@@ -389,16 +417,12 @@ int upload_lattice(hlb_lattice* h) {
   HLB_CHECK(h->bii.alloc((size_t)SB));
   HLB_CHECK(h->bq.alloc((size_t)18 * SB));
   HLB_CHECK(h->bnrm.alloc((size_t)3 * SB));
-  HLB_CHECK(h->G.alloc((size_t)Q * SI));
   HLB_CHECK(h->ladd_s.alloc((size_t)18 * SI));
   HLB_CHECK(h->rho.alloc((size_t)S));
   HLB_CHECK(h->u.alloc((size_t)3 * S));
   HLB_CHECK(h->wss.alloc((size_t)S));
   HLB_CHECK(h->flag.alloc(1));
   HLB_CHECK(h->feq_vals.alloc(Q));
-  HLB_CHECK(hipMemsetAsync(h->A[0].p, 0, sizeof(double) * Q * S, h->stream));
-  HLB_CHECK(hipMemsetAsync(h->A[1].p, 0, sizeof(double) * Q * S, h->stream));
-  HLB_CHECK(hipMemsetAsync(h->G.p, 0, sizeof(double) * Q * SI, h->stream));
 
   // Pull table in device slots, flag on plane 0, compact boundary index.
   {
@@ -495,7 +519,8 @@ int validate_params(const hlb_params& p, const hlb_iolet_desc* in, int nin, cons
   if (p.wall_bc != HLB_WALL_SBB && p.wall_bc != HLB_WALL_BFL) return fail(HLB_EINVAL, "hlb_create: unknown wall BC");
   for (int bc : {p.inlet_bc, p.outlet_bc})
     if (bc != HLB_IOLET_NASH && bc != HLB_IOLET_LADD) return fail(HLB_EINVAL, "hlb_create: unknown iolet BC");
-  if (p.precision != HLB_FP64) return fail(HLB_EINVAL, "hlb_create: only HLB_FP64 storage is available");
+  if (p.precision != HLB_FP64 && p.precision != HLB_FP32_SHIFTED)
+    return fail(HLB_EINVAL, "hlb_create: unknown storage precision");
   if (!(p.tau > 0.5)) return fail(HLB_EINVAL, "hlb_create: tau must exceed 1/2");
   for (int side = 0; side < 2; ++side) {
     const hlb_iolet_desc* v = side ? out : in;
@@ -520,6 +545,66 @@ int validate_params(const hlb_params& p, const hlb_iolet_desc* in, int nin, cons
   return HLB_OK;
 }
 
+// The member record of an FP32 handle for a launch sequence starting now.
+Member32 member_record(const hlb_lattice* h, int props_at) {
+  Member32 m{};
+  m.A[0] = h->A32[0].p;
+  m.A[1] = h->A32[1].p;
+  m.F = h->F32.p;
+  m.pull = h->pull.p;
+  m.bidx = h->bidx.p;
+  m.bwall = h->bwall.p;
+  m.biolet = h->biolet.p;
+  m.bgi = h->bgi.p;
+  m.bii = h->bii.p;
+  m.bq = h->bq.p;
+  m.bnrm = h->bnrm.p;
+  m.G = h->G32.p;
+  m.ladd_s = h->ladd_s.p;
+  m.rho = h->rho.p;
+  m.u = h->u.p;
+  m.wss = h->wss.p;
+  m.io_nf = h->io_nf.p;
+  m.io_n = h->io_n.p;
+  m.io_dens = h->io_dens.p;
+  m.io_vmax = h->io_vmax.p;
+  m.wk_bit = h->wk_bit.p;
+  m.wk_contrib = h->wk_contrib.p;
+  m.N = h->L.N;
+  m.S = h->S;
+  m.SB = h->SB;
+  m.SI = h->SI;
+  m.nio = std::max<int>(1, (int)h->iolets.size());
+  m.n_in = h->n_in;
+  m.kernel = h->params.kernel;
+  m.wall_bc = h->params.wall_bc;
+  m.inlet_bc = h->params.inlet_bc;
+  m.outlet_bc = h->params.outlet_bc;
+  m.cur0 = h->cur;
+  m.first0 = h->have_post ? 0 : 1;
+  m.props_at = props_at;
+  m.wg_begin = 0;
+  m.tau = (float)h->params.tau;
+  m.omega = (float)h->omega;
+  m.half_omega = (float)h->half_omega;
+  m.half_omega_minus = (float)h->half_omega_minus;
+  m.les_k = (float)h->les_k;
+  return m;
+}
+
+uint32_t member_grid(int64_t N) {
+  const uint32_t tiles = (uint32_t)((N + 255) / 256);
+  return (tiles + 7) / 8 * 8;
+}
+
+// Writes the member record through the pinned staging slot; the caller has
+// synchronised the stream, so the slot is free.
+int upload_member(hlb_lattice* h, int props_at) {
+  *h->h_mem = member_record(h, props_at);
+  HLB_CHECK(hipMemcpyAsync(h->mem.p, h->h_mem, sizeof(Member32), hipMemcpyHostToDevice, h->stream));
+  return HLB_OK;
+}
+
 // Stages the per-step iolet values of steps t .. t+n-1 for the device.
 int stage_iolet_values(hlb_lattice* h, int64_t t, int64_t n) {
   const int nio = std::max<int>(1, (int)h->iolets.size());
@@ -537,6 +622,25 @@ int stage_iolet_values(hlb_lattice* h, int64_t t, int64_t n) {
 // Enqueues the launches of n steps whose iolet values are staged at rows 0..n-1.
 int enqueue_steps(hlb_lattice* h, int64_t n, bool props_last) {
   const int nio = std::max<int>(1, (int)h->iolets.size());
+  if (h->fp32()) {
+    const bool first0 = !h->have_post;
+    int rc = upload_member(h, props_last ? (int)(n - 1) : -1);
+    if (rc != HLB_OK) return rc;
+    const uint32_t grid = member_grid(h->L.N);
+    for (int64_t s = 0; s < n; ++s) {
+      if (h->any_wk)
+        HLB_CHECK(launch_wk_begin(h->wk.p, (int)h->iolets.size(), h->io_dens.p + s * nio, h->stream));
+      HLB_CHECK(launch_step_fp32(h->mem.p, nullptr, grid, (int)s, h->stream));
+      if (h->any_wk)
+        HLB_CHECK(launch_wk_end(h->wk.p, (int)h->iolets.size(), h->wk_contrib.p, h->wk_list_d.p, h->stream));
+    }
+    if (n > 0) {
+      h->cur = first0 ? (h->cur ^ (int)((n - 1) & 1)) : (h->cur ^ (int)(n & 1));
+      h->have_post = true;
+    }
+    h->t += n;
+    return HLB_OK;
+  }
   for (int64_t s = 0; s < n; ++s) {
     const bool first = !h->have_post;
     const bool props = props_last && (s == n - 1);
@@ -578,6 +682,14 @@ int run_steps(hlb_lattice* h, int64_t nsteps, uint32_t flags) {
 // Pre-collision f entering step t into F (device order, SoA).
 int materialise_f(hlb_lattice* h) {
   if (!h->have_post) return HLB_OK;  // F already holds it
+  if (h->fp32()) {
+    if (!h->F32.p) HLB_CHECK(h->F32.alloc((size_t)Q * h->S));
+    HLB_CHECK(hipStreamSynchronize(h->stream));
+    int rc = upload_member(h, -1);
+    if (rc != HLB_OK) return rc;
+    HLB_CHECK(launch_gather_fp32(h->mem.p, h->L.N, h->F32.p, h->stream));
+    return HLB_OK;
+  }
   if (!h->F.p) HLB_CHECK(h->F.alloc((size_t)Q * h->S));
   StepArgs a = make_args(h, h->A[h->cur].p, nullptr);
   HLB_CHECK(launch_gather_fp64(a, h->F.p, h->S, h->stream));
@@ -681,6 +793,18 @@ extern "C" int hlb_init_equilibrium(hlb_handle h, double rho0, int64_t t0) {
   double feq[Q];
   const FeqPre p = feq_pre(rho0, 0.0, 0.0, 0.0);
   feq_all(p, 0.0, 0.0, 0.0, feq);
+  if (h->fp32()) {
+    float g[Q];
+    for (int j = 0; j < Q; ++j) g[j] = (float)(feq[j] - Wt(j));
+    if (!h->F32.p) HLB_CHECK(h->F32.alloc((size_t)Q * h->S));
+    HLB_CHECK(hipMemcpyAsync(h->feq32.p, g, sizeof g, hipMemcpyHostToDevice, h->stream));
+    HLB_CHECK(launch_fill_fp32(h->F32.p, h->L.N, h->S, h->feq32.p, h->stream));
+    HLB_CHECK(hipStreamSynchronize(h->stream));
+    h->have_post = false;
+    h->f_valid = true;
+    h->t = t0;
+    return HLB_OK;
+  }
   if (!h->F.p) HLB_CHECK(h->F.alloc((size_t)Q * h->S));
   HLB_CHECK(hipMemcpyAsync(h->feq_vals.p, feq, sizeof feq, hipMemcpyHostToDevice, h->stream));
   HLB_CHECK(launch_fill_fp64(h->F.p, h->L.N, h->S, h->feq_vals.p, h->stream));
@@ -699,6 +823,19 @@ extern "C" int hlb_upload_f(hlb_handle h, const double* f, int64_t t0) {
     if (rc != HLB_OK) return rc;
   }
   const int64_t N = h->L.N, S = h->S;
+  if (h->fp32()) {
+    std::vector<float> soa((size_t)Q * S, 0.0f);
+    for (int64_t d = 0; d < N; ++d) {
+      const double* src = f + (size_t)h->perm[d] * Q;
+      for (int j = 0; j < Q; ++j) soa[(size_t)j * S + d] = (float)(src[j] - Wt(j));
+    }
+    if (!h->F32.p) HLB_CHECK(h->F32.alloc((size_t)Q * S));
+    HLB_CHECK(hipMemcpy(h->F32.p, soa.data(), sizeof(float) * soa.size(), hipMemcpyHostToDevice));
+    h->have_post = false;
+    h->f_valid = true;
+    h->t = t0;
+    return HLB_OK;
+  }
   std::vector<double> soa((size_t)Q * S, 0.0);
   for (int64_t d = 0; d < N; ++d) {
     const double* src = f + (size_t)h->perm[d] * Q;
@@ -748,6 +885,16 @@ extern "C" int hlb_download_f(hlb_handle h, double* f) {
   int rc = materialise_f(h);
   if (rc != HLB_OK) return rc;
   const int64_t N = h->L.N, S = h->S;
+  if (h->fp32()) {
+    std::vector<float> soa((size_t)Q * S);
+    HLB_CHECK(hipMemcpyAsync(soa.data(), h->F32.p, sizeof(float) * soa.size(), hipMemcpyDeviceToHost, h->stream));
+    HLB_CHECK(hipStreamSynchronize(h->stream));
+    for (int64_t d = 0; d < N; ++d) {
+      double* dst = f + (size_t)h->perm[d] * Q;
+      for (int j = 0; j < Q; ++j) dst[j] = (double)soa[(size_t)j * S + d] + Wt(j);
+    }
+    return HLB_OK;
+  }
   std::vector<double> soa((size_t)Q * S);
   HLB_CHECK(hipMemcpyAsync(soa.data(), h->F.p, sizeof(double) * soa.size(), hipMemcpyDeviceToHost, h->stream));
   HLB_CHECK(hipStreamSynchronize(h->stream));
@@ -788,7 +935,8 @@ extern "C" int hlb_stability(hlb_handle h, int* stable) {
   int rc = materialise_f(h);
   if (rc != HLB_OK) return rc;
   HLB_CHECK(hipMemsetAsync(h->flag.p, 0, sizeof(int), h->stream));
-  HLB_CHECK(launch_unstable_fp64(h->F.p, h->L.N, h->S, h->flag.p, h->stream));
+  if (h->fp32()) HLB_CHECK(launch_unstable_fp32(h->F32.p, h->L.N, h->S, h->flag.p, h->stream));
+  else HLB_CHECK(launch_unstable_fp64(h->F.p, h->L.N, h->S, h->flag.p, h->stream));
   int flag = 0;
   HLB_CHECK(hipMemcpyAsync(&flag, h->flag.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HLB_CHECK(hipStreamSynchronize(h->stream));

@@ -1,0 +1,386 @@
+// lbm_fp32.hip -- the FP32-shifted collide+stream pass for gfx950.
+//
+// Same pull step as lbm_fp64.hip (gather the pre-collision state the
+// reference holds in fOld, apply the boundary delegates, collide, store the
+// post-collision row), with the distributions stored and evolved as
+//   g_i = f_i - w_i rho0,   rho0 = 1,
+// in float.  Shifting removes the O(w_i) constant part, so float keeps ~7
+// significant digits of the deviation from rest instead of of f itself.  In
+// shifted form
+//   rho = rho0 + sum g,  j = sum c g,
+//   g_eq_i = w_i (drho + 3 c.j + 4.5 (c.j)^2 / rho - 1.5 |j|^2 / rho),
+// the compressible equilibrium of Lattice.h:374-448 minus w_i rho0; the
+// delegates (SBB, BFL, Nash, Ladd) are affine with weights summing to one, so
+// they apply to g unchanged.  Iolet densities and speeds arrive in double,
+// Windkessel flux terms and the output fields are written in double.
+//
+// A launch advances every member of a table (see Member32): workgroup tile t
+// belongs to member wg_member[t]; a single lattice is a table of one.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "../../include/hemepure_amd.h"
+#include "d3q19.h"
+#include "device_util.h"
+#include "lbm_kernels.h"
+
+namespace hlb {
+
+namespace {
+
+constexpr float kW0 = 1.0f / 3.0f, kW1 = 1.0f / 18.0f, kW2 = 1.0f / 36.0f;
+
+__device__ __forceinline__ constexpr float wf(int j) { return j == 0 ? kW0 : (j <= 6 ? kW1 : kW2); }
+
+// Shifted moments: drho = sum g, j = sum c g.
+__device__ __forceinline__ void moments32(const float* g, float& drho, float& jx, float& jy, float& jz) {
+  drho = g[0];
+  jx = 0.0f;
+  jy = 0.0f;
+  jz = 0.0f;
+#pragma unroll
+  for (int k = 1; k < Q; ++k) {
+    drho += g[k];
+    if (CX[k] > 0) jx += g[k];
+    if (CX[k] < 0) jx -= g[k];
+    if (CY[k] > 0) jy += g[k];
+    if (CY[k] < 0) jy -= g[k];
+    if (CZ[k] > 0) jz += g[k];
+    if (CZ[k] < 0) jz -= g[k];
+  }
+}
+
+template <int K>
+__device__ __forceinline__ float cdotf(float jx, float jy, float jz) {
+  float r = 0.0f;
+  if (CX[K] > 0) r += jx;
+  if (CX[K] < 0) r -= jx;
+  if (CY[K] > 0) r += jy;
+  if (CY[K] < 0) r -= jy;
+  if (CZ[K] > 0) r += jz;
+  if (CZ[K] < 0) r -= jz;
+  return r;
+}
+
+// One shifted equilibrium component; inv = 1 / rho, usq15 = 1.5 |j|^2 / rho.
+template <int K>
+__device__ __forceinline__ float geq_dir(float drho, float inv, float usq15, float jx, float jy, float jz) {
+  const float cu = cdotf<K>(jx, jy, jz);
+  return wf(K) * (drho + 3.0f * cu + 4.5f * cu * cu * inv - usq15);
+}
+
+template <int K = 0>
+__device__ __forceinline__ void geq_all(float drho, float inv, float usq15, float jx, float jy, float jz,
+                                        float* geq) {
+  if constexpr (K < Q) {
+    geq[K] = geq_dir<K>(drho, inv, usq15, jx, jy, jz);
+    geq_all<K + 1>(drho, inv, usq15, jx, jy, jz, geq);
+  }
+}
+
+struct Sym3f {
+  float xx, yx, yy, zx, zy, zz;
+};
+
+__device__ __forceinline__ Sym3f second_moments32(const float* v) {
+  Sym3f s{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 1; k < Q; ++k) {
+    const float cx = (float)CX[k], cy = (float)CY[k], cz = (float)CZ[k];
+    s.xx += cx * cx * v[k];
+    s.yx += cy * cx * v[k];
+    s.yy += cy * cy * v[k];
+    s.zx += cz * cx * v[k];
+    s.zy += cz * cy * v[k];
+    s.zz += cz * cz * v[k];
+  }
+  return s;
+}
+
+// Which A plane set the step reads and writes: k = step index counted from
+// the first gathered step; k < 0 is the first step after an initial state.
+struct Planes {
+  const float* in;
+  float* out;
+  bool first;
+};
+
+__device__ __forceinline__ Planes member_planes(const Member32& m, int step) {
+  const int k = m.first0 ? step - 1 : step;
+  Planes p;
+  p.first = k < 0;
+  const int c = k < 0 ? m.cur0 : (m.cur0 ^ (k & 1));
+  p.in = k < 0 ? m.F : m.A[c];
+  p.out = k < 0 ? m.A[m.cur0] : m.A[c ^ 1];
+  return p;
+}
+
+__device__ __forceinline__ void gather_pull32(const Member32& m, const float* in, uint32_t d, uint32_t raw1,
+                                              float* g) {
+  uint32_t src[Q];
+  src[1] = raw1 & kIndexMask;
+#pragma unroll
+  for (int j = 2; j < Q; ++j) src[j] = ldp_nt(plane(m.pull, m.S, j - 1), d);
+  g[0] = ldp(in, d);
+#pragma unroll
+  for (int j = 1; j < Q; ++j) g[j] = ldp(plane(in, m.S, j), src[j]);
+}
+
+// The boundary delegates of one flagged site, as boundary_fixup in the FP64
+// pass (SimpleBounceBackDelegate.h:36-44, BouzidiFirdaousLallemandDelegate.h:
+// 44-93, iolet values from G).
+__device__ __forceinline__ void boundary_fixup32(const Member32& m, const float* in, uint32_t d, uint32_t b,
+                                                 float* g) {
+  const uint32_t wm = ldp(m.bwall, b), im = ldp(m.biolet, b);
+  float own[Q];
+#pragma unroll
+  for (int j = 1; j < Q; ++j) own[j] = ldp(plane(in, m.S, j), d);
+  const bool bfl = m.wall_bc == HLB_WALL_BFL;
+  float q[Q];
+  if (bfl && wm != 0) {
+#pragma unroll
+    for (int k = 1; k < Q; ++k) q[k] = ldp(m.bq + (int64_t)(k - 1) * m.SB, b);
+  }
+  if (im != 0) {
+    const uint32_t i = (uint32_t)ldp(m.bii, b);
+#pragma unroll
+    for (int j = 1; j < Q; ++j)
+      if (im & (1u << (INV[j] - 1))) g[j] = ldp(plane(m.G, m.SI, j), i);
+  }
+  if (wm != 0) {
+#pragma unroll
+    for (int j = 1; j < Q; ++j) {
+      const int ib = INV[j];
+      const uint32_t bit = 1u << (ib - 1), bitJ = 1u << (j - 1);
+      if ((wm & bit) && !(im & bit)) {
+        const float qd = bfl ? q[ib] : 0.0f;
+        if (!bfl || (wm & bitJ) || qd < 0.5f) g[j] = own[ib];
+        else g[j] = (own[ib] + (2.0f * qd - 1.0f) * own[j]) / (2.0f * qd);
+      }
+    }
+    if (bfl) {
+#pragma unroll
+      for (int k = 1; k < Q; ++k) {
+        const int j = INV[k];
+        const float qk = q[k];
+        if ((wm & (1u << (k - 1))) && !(wm & (1u << (j - 1))) && qk < 0.5f)
+          g[j] = 2.0f * qk * g[j] + (1.0f - 2.0f * qk) * g[k];
+      }
+    }
+  }
+}
+
+// Nash ghost equilibrium / Ladd moving-wall bounce-back for each iolet link
+// (NashZerothOrderPressureDelegate.h:36-76, LaddIoletDelegate.h:37-86), in
+// shifted form.
+template <int D = 1>
+__device__ __forceinline__ void iolet_links32(const Member32& m, const double* dens, const double* vmax,
+                                              uint32_t i, uint32_t im, int gi, int bc, float rho, float vx,
+                                              float vy, float vz, const float* gpost) {
+  if constexpr (D < Q) {
+    constexpr int U = INV[D];
+    if (im & (1u << (D - 1))) {
+      float val;
+      if (bc == HLB_IOLET_NASH) {
+        const double rgd = ldp(dens, (uint32_t)gi);
+        const float rg = (float)rgd, drg = (float)(rgd - 1.0);
+        const float nx = ldp(m.io_nf, 3u * gi), ny = ldp(m.io_nf, 3u * gi + 1), nz = ldp(m.io_nf, 3u * gi + 2);
+        const float component = vx * nx + vy * ny + vz * nz;
+        const float gx = nx * component * rg, gy = ny * component * rg, gz = nz * component * rg;
+        const float inv = 1.0f / rg;
+        const float usq15 = 1.5f * (gx * gx + gy * gy + gz * gz) * inv;
+        val = geq_dir<U>(drg, inv, usq15, gx, gy, gz);
+      } else {
+        const float s = (float)(ldp(vmax, (uint32_t)gi) * ldp(m.ladd_s + (int64_t)(D - 1) * m.SI, i));
+        const float wx = (float)ldp(m.io_n, 3u * gi) * s * rho, wy = (float)ldp(m.io_n, 3u * gi + 1) * s * rho,
+                    wz = (float)ldp(m.io_n, 3u * gi + 2) * s * rho;
+        const float correction = 2.0f * wf(D) * cdotf<D>(wx, wy, wz) * 3.0f;
+        val = gpost[D] - correction;
+      }
+      stp(plane(m.G, m.SI, U), i, val);
+    }
+    iolet_links32<D + 1>(m, dens, vmax, i, im, gi, bc, rho, vx, vy, vz, gpost);
+  }
+}
+
+// Wall shear stress magnitude (Lattice.h:1122-1156) from gpost - g.
+__device__ __forceinline__ float wall_shear_stress32(const Member32& m, uint32_t b, float tau, const float* g,
+                                                     const float* gpost) {
+  float dg[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) dg[j] = gpost[j] - g[j];
+  const Sym3f s = second_moments32(dg);
+  const float fac = -(tau - 0.5f);
+  const float p00 = s.xx * fac, p10 = s.yx * fac, p11 = s.yy * fac;
+  const float p20 = s.zx * fac, p21 = s.zy * fac, p22 = s.zz * fac;
+  const float n0 = ldp(m.bnrm, b), n1 = ldp(m.bnrm + m.SB, b), n2 = ldp(m.bnrm + 2 * m.SB, b);
+  const float sv0 = p00 * n0 + p10 * n1 + p20 * n2;
+  const float sv1 = p10 * n0 + p11 * n1 + p21 * n2;
+  const float sv2 = p20 * n0 + p21 * n1 + p22 * n2;
+  const float sq = sv0 * sv0 + sv1 * sv1 + sv2 * sv2;
+  const float nn = sv0 * n0 + sv1 * n1 + sv2 * n2;
+  return __builtin_sqrtf(sq - nn * nn);
+}
+
+// The member of a tile, made wave-uniform so the record is read with scalar
+// loads.
+__device__ __forceinline__ int member_of_tile(const int32_t* wg_member, uint32_t tile) {
+  return wg_member ? __builtin_amdgcn_readfirstlane(ldp(wg_member, tile)) : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void step_fp32(const Member32* __restrict__ members,
+                                                    const int32_t* __restrict__ wg_member, int step) {
+  const uint32_t tile = xcd_tile();
+  const int mi = member_of_tile(wg_member, tile);
+  if (mi < 0) return;  // padding tile
+  const Member32& m = members[mi];
+  const uint32_t d = (tile - m.wg_begin) * kBlock + threadIdx.x;
+  if (d >= (uint32_t)m.N) return;
+  const Planes P = member_planes(m, step);
+  const uint32_t raw1 = ldp_nt(m.pull, d);
+  const bool isB = (raw1 & kBoundaryFlag) != 0;
+  float g[Q];
+  if (P.first) {
+#pragma unroll
+    for (int j = 0; j < Q; ++j) g[j] = ldp(plane(P.in, m.S, j), d);
+  } else {
+    gather_pull32(m, P.in, d, raw1, g);
+    if (isB) boundary_fixup32(m, P.in, d, (uint32_t)ldp(m.bidx, d), g);
+  }
+
+  float drho, jx, jy, jz;
+  moments32(g, drho, jx, jy, jz);
+  const float rho = 1.0f + drho;
+  const float inv = 1.0f / rho;
+  const float ux = jx * inv, uy = jy * inv, uz = jz * inv;
+  const float usq15 = 1.5f * (jx * jx + jy * jy + jz * jz) * inv;
+  float geq[Q];
+  geq_all(drho, inv, usq15, jx, jy, jz, geq);
+  float gneq[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) gneq[j] = g[j] - geq[j];
+  float tau = m.tau;
+  float gpost[Q];
+  if (m.kernel == HLB_KERNEL_TRT) {
+    gpost[0] = g[0] + m.omega * gneq[0];
+#pragma unroll
+    for (int i = 1; i < Q; i += 2) {
+      const float sym = m.half_omega * (gneq[i] + gneq[i + 1]);
+      const float asym = m.half_omega_minus * (gneq[i] - gneq[i + 1]);
+      gpost[i] = g[i] + sym + asym;
+      gpost[i + 1] = g[i + 1] + sym - asym;
+    }
+  } else {
+    float om = m.omega;
+    if (m.kernel == HLB_KERNEL_LES) {
+      const Sym3f s = second_moments32(gneq);
+      const float q2 = s.xx * s.xx + 2.0f * s.yx * s.yx + 2.0f * s.zx * s.zx + s.yy * s.yy +
+                       2.0f * s.zy * s.zy + s.zz * s.zz;
+      tau = 0.5f * (m.tau + __builtin_sqrtf(m.tau * m.tau + m.les_k * __builtin_sqrtf(q2)));
+      om = -1.0f / tau;
+    }
+#pragma unroll
+    for (int j = 0; j < Q; ++j) gpost[j] = g[j] + gneq[j] * om;
+  }
+
+#pragma unroll
+  for (int j = 0; j < Q; ++j) stp(plane(P.out, m.S, j), d, gpost[j]);
+
+  const bool props = step == m.props_at;
+  float w = FLT_MAX;
+  if (isB) {
+    const uint32_t b = (uint32_t)ldp(m.bidx, d);
+    const int32_t ii = ldp(m.bii, b);
+    if (ii >= 0) {
+      const uint32_t im = ldp(m.biolet, b);
+      const int gi = ldp(m.bgi, b);
+      const int bc = gi < m.n_in ? m.inlet_bc : m.outlet_bc;
+      const double* dens = m.io_dens + (int64_t)step * m.nio;
+      const double* vmax = m.io_vmax + (int64_t)step * m.nio;
+      iolet_links32(m, dens, vmax, (uint32_t)ii, im, gi, bc, rho, ux, uy, uz, gpost);
+      if (im & ldp(m.wk_bit, (uint32_t)gi)) {
+        const double c = (double)ux * -ldp(m.io_n, 3u * gi) + (double)uy * -ldp(m.io_n, 3u * gi + 1) +
+                         (double)uz * -ldp(m.io_n, 3u * gi + 2);
+        stp(m.wk_contrib, (uint32_t)ii, c);
+      }
+    }
+    if (props && ldp(m.bwall, b) != 0) w = wall_shear_stress32(m, b, tau, g, gpost);
+  }
+  if (props) {
+    stp(m.rho, d, 1.0 + (double)drho);
+    stp(m.u, d, (double)ux);
+    stp(plane(m.u, m.S, 1), d, (double)uy);
+    stp(plane(m.u, m.S, 2), d, (double)uz);
+    stp(m.wss, d, w == FLT_MAX ? DBL_MAX : (double)w);
+  }
+}
+
+// Pre-collision g entering the next step into F (device order), for
+// downloads and the stability scan.
+__global__ __launch_bounds__(kBlock) void gather_fp32(const Member32* __restrict__ members, float* __restrict__ F) {
+  const Member32& m = members[0];
+  const uint32_t d = site_index();
+  if (d >= (uint32_t)m.N) return;
+  const float* in = m.A[m.cur0];
+  const uint32_t raw1 = ldp(m.pull, d);
+  float g[Q];
+  gather_pull32(m, in, d, raw1, g);
+  if (raw1 & kBoundaryFlag) boundary_fixup32(m, in, d, (uint32_t)ldp(m.bidx, d), g);
+#pragma unroll
+  for (int j = 0; j < Q; ++j) stp(plane(F, m.S, j), d, g[j]);
+}
+
+__global__ __launch_bounds__(kBlock) void fill_planes_fp32(float* __restrict__ F, int64_t N, int64_t S,
+                                                           const float* __restrict__ vals) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= N) return;
+#pragma unroll
+  for (int j = 0; j < Q; ++j) F[(int64_t)j * S + k] = vals[j];
+}
+
+// StabilityTester on the shifted storage: f = g + w > 0.
+__global__ __launch_bounds__(kBlock) void unstable_fp32(const float* __restrict__ F, int64_t N, int64_t S,
+                                                        int* __restrict__ flag) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= N) return;
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < Q; ++j) bad |= !(F[(int64_t)j * S + k] > -wf(j));
+  if (bad) *flag = 1;
+}
+
+}  // namespace
+
+// Grid of a member-table launch: `tiles` workgroup tiles padded to a multiple
+// of the XCD count (padding tiles map to member -1).
+hipError_t launch_step_fp32(const Member32* members, const int32_t* wg_member, uint32_t grid, int step,
+                            hipStream_t s) {
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(step_fp32, dim3(grid), dim3(kBlock), 0, s, members, wg_member, step);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_fp32(const Member32* members, int64_t N, float* F, hipStream_t s) {
+  if (N == 0) return hipSuccess;
+  const uint32_t tiles = (uint32_t)((N + kBlock - 1) / kBlock);
+  const uint32_t grid = (tiles + kXcds - 1) / kXcds * kXcds;
+  hipLaunchKernelGGL(gather_fp32, dim3(grid), dim3(kBlock), 0, s, members, F);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_fp32(float* F, int64_t N, int64_t S, const float* vals, hipStream_t s) {
+  if (N == 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_planes_fp32, dim3((unsigned)((N + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, F, N, S,
+                     vals);
+  return hipGetLastError();
+}
+
+hipError_t launch_unstable_fp32(const float* F, int64_t N, int64_t S, int* flag, hipStream_t s) {
+  if (N == 0) return hipSuccess;
+  hipLaunchKernelGGL(unstable_fp32, dim3((unsigned)((N + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, F, N, S,
+                     flag);
+  return hipGetLastError();
+}
+
+}  // namespace hlb

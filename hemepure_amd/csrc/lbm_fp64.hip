@@ -24,39 +24,12 @@
 
 #include "../../include/hemepure_amd.h"
 #include "d3q19.h"
+#include "device_util.h"
 #include "lbm_kernels.h"
 
 namespace hlb {
 
 namespace {
-
-constexpr int kBlock = 256;
-constexpr int kXcds = 8;
-
-// Loads and stores through a uniform plane base and a 32-bit element index:
-// the byte offset stays 32-bit (planes are < 2^29 elements, checked at
-// create time), so the address is an SGPR base plus one VGPR offset.
-template <typename T>
-__device__ __forceinline__ T ldp(const T* base, uint32_t i) {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
-}
-template <typename T>
-__device__ __forceinline__ T ldp_nt(const T* base, uint32_t i) {
-  return __builtin_nontemporal_load(
-      reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T)));
-}
-template <typename T>
-__device__ __forceinline__ void stp(T* base, uint32_t i, T v) {
-  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T)) = v;
-}
-
-__device__ __forceinline__ double* plane(double* p, int64_t S, int j) { return p + (int64_t)j * S; }
-
-__device__ __forceinline__ uint32_t site_index() {
-  const uint32_t per = gridDim.x / kXcds;
-  const uint32_t wg = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
-  return wg * kBlock + threadIdx.x;
-}
 
 // Gathers along the pull table; raw1 is plane 0 with the boundary flag.
 __device__ __forceinline__ void gather_pull(const StepArgs& a, uint32_t d, uint32_t raw1, double* f) {
@@ -327,13 +300,27 @@ __global__ void wk_begin(WkIolet* __restrict__ wk, int n, double* __restrict__ d
 
 // End of a step: the coupling sites' terms summed in the reference's site
 // order, the centre site's pressure update (InOutLetWK2.cc:82-95,
-// InOutLetWK3.cc:70-95) and the post-stream commit (:97-106).
-__global__ void wk_end(WkIolet* __restrict__ wk, int n, const double* __restrict__ contrib,
-                       const int32_t* __restrict__ list) {
-  const int g = threadIdx.x;
-  if (g >= n) return;
-  WkIolet& w = wk[g];
-  if (w.kind != HLB_IOLET_WK2 && w.kind != HLB_IOLET_WK3) return;
+// InOutLetWK3.cc:70-95) and the post-stream commit (:97-106).  One workgroup
+// per iolet: the lanes stage the terms in LDS, lane 0 adds them in order (the
+// reference's sequential flowRateNew += rounds exactly so).
+constexpr int kWkStage = 2048;
+
+__global__ __launch_bounds__(256) void wk_end(WkIolet* __restrict__ wk, const double* __restrict__ contrib,
+                                              const int32_t* __restrict__ list) {
+  __shared__ double stage[kWkStage];
+  WkIolet& w = wk[blockIdx.x];
+  if (w.kind != HLB_IOLET_WK2 && w.kind != HLB_IOLET_WK3) return;  // uniform per workgroup
+  const int32_t lb = w.list_begin, le = w.list_end;
+  double sum = w.flowRateNew;
+  for (int32_t c0 = lb; c0 < le; c0 += kWkStage) {
+    const int32_t cn = min(kWkStage, le - c0);
+    for (int32_t e = threadIdx.x; e < cn; e += blockDim.x) stage[e] = contrib[list[c0 + e]];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int32_t e = 0; e < cn; ++e) sum += stage[e];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
   if (w.centre_coupled) {
     const double pressure = w.density * CS2;
     if (w.kind == HLB_IOLET_WK2) {
@@ -348,10 +335,8 @@ __global__ void wk_end(WkIolet* __restrict__ wk, int n, const double* __restrict
       w.densityNew = pressureNew / CS2;
     }
   }
-  double sum = w.flowRateNew;
-  for (int32_t e = w.list_begin; e < w.list_end; ++e) sum += contrib[list[e]];
   w.flowRateNew = sum;
-  w.siteCount += (double)(w.list_end - w.list_begin);
+  w.siteCount += (double)(le - lb);
   if (w.centre_coupled) {
     w.density = w.densityNew;
     w.flowRateOld = w.flowRate;
@@ -366,7 +351,8 @@ hipError_t launch_wk_begin(WkIolet* wk, int n, double* dens_row, hipStream_t s) 
 }
 
 hipError_t launch_wk_end(WkIolet* wk, int n, const double* contrib, const int32_t* list, hipStream_t s) {
-  hipLaunchKernelGGL(wk_end, dim3(1), dim3(kMaxIolets), 0, s, wk, n, contrib, list);
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(wk_end, dim3(n), dim3(256), 0, s, wk, contrib, list);
   return hipGetLastError();
 }
 

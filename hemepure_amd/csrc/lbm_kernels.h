@@ -70,6 +70,44 @@ struct StepArgs {
   double les_k;       // 18 sqrt(2) C_s^2 evaluated in the reference's order
 };
 
+// One lattice of the FP32-shifted form.  The storage holds g = f - w rho0
+// (rho0 = 1) in float; a launch advances every member of a table, one
+// workgroup tile of 256 device slots at a time, so a single lattice and a
+// batch of lattices of uneven size run the same kernel.  Layout per member is
+// the FP64 one with float planes for A, F and G.
+struct Member32 {
+  float* A[2];            // post-collision g, 19 planes of S
+  const float* F;         // pre-collision g (initial state / checkpoint)
+  const uint32_t* pull;   // 18 planes of S, boundary flag on plane 0
+  const int32_t* bidx;
+  const uint32_t* bwall;
+  const uint32_t* biolet;
+  const int32_t* bgi;
+  const int32_t* bii;
+  const float* bq;
+  const float* bnrm;
+  float* G;               // iolet values for the next step, 19 planes of SI
+  const double* ladd_s;
+  double* rho;
+  double* u;
+  double* wss;
+  const float* io_nf;
+  const double* io_n;
+  const double* io_dens;  // rows of nio per step of the launch sequence
+  const double* io_vmax;
+  const uint32_t* wk_bit;
+  double* wk_contrib;
+  int64_t N, S, SB, SI;
+  int32_t nio, n_in;
+  int32_t kernel, wall_bc, inlet_bc, outlet_bc;
+  int32_t cur0;           // A[cur0] holds the latest values when the sequence starts
+  int32_t first0;         // 1: step 0 of the sequence reads F (pre-collision)
+  int32_t props_at;       // step of the sequence that writes rho, u, WSS (-1: none)
+  uint32_t wg_begin;      // first workgroup tile of this member in the launch
+  float tau, omega, half_omega, half_omega_minus, les_k;
+  int32_t pad_;
+};
+
 // Windkessel state of one iolet (InOutLetWK2.cc:13-18, InOutLetWK3.cc:13-18)
 // plus its constants; updated on the device between steps.
 struct WkIolet {

@@ -137,9 +137,9 @@ class Case:
     inlets: list = field(default_factory=list)
     outlets: list = field(default_factory=list)
 
-    def product(self) -> H.Lattice:
+    def product(self, precision=H.FP64) -> H.Lattice:
         prm = H.params(kernel=self.kernel, wall_bc=self.wall_bc, inlet_bc=self.inlet_bc,
-                       outlet_bc=self.outlet_bc, tau=self.tau, trt_lambda=self.trt_lambda)
+                       outlet_bc=self.outlet_bc, tau=self.tau, trt_lambda=self.trt_lambda, precision=precision)
         ins = [H.iolet_desc(s.kind, **s.kw()) for s in self.inlets]
         outs = [H.iolet_desc(s.kind, **s.kw()) for s in self.outlets]
         return H.Lattice(self.geom, prm, ins, outs)
