@@ -7,6 +7,10 @@ radius jitter), TRT with magic parameter 3/16 and tau 0.6, BFL walls, a Ladd
 parabolic velocity inlet driven by a sinusoidal peak-speed table (peak 0.05,
 period 4000) and two Nash pressure outlets at rho = 1, FP64 storage.
 
+The same workload is then timed in the FP32-shifted storage form (float
+g = f - w rho0, 19 x 4 B read + written + 18 pull indices = 224 B per site
+update), reported under "forms".
+
 Prints one JSON line (rank 0, N = 1).  MFLUPS = sites x steps / seconds / 1e6.
 The roofline object prices the step kernel against MI355X's 8 TB/s HBM with
 the algorithmic byte count per site update (FP64: 19 reads + 19 writes of 8 B
@@ -31,7 +35,7 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "tests"))
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_SITE = {"f64": 19 * 8 * 2 + 18 * 4}
+BYTES_PER_SITE = {"f64": 19 * 8 * 2 + 18 * 4, "f32": 19 * 4 * 2 + 18 * 4}
 
 
 def log(msg: str) -> None:
@@ -147,6 +151,30 @@ def _committed_traffic(workload: str, n: int):
     return int(m_bytes.group(1))
 
 
+def time_steps(lat, steps: int):
+    """Milliseconds of `steps` steps, from HIP events on the lattice's stream."""
+    ms_total = 0.0
+    left = steps
+    while left > 0:
+        chunk = min(left, 1024)
+        ms_total += lat.step_timed(chunk)
+        left -= chunk
+    return ms_total
+
+
+def time_form(case, precision, steps: int, warmup: int):
+    lat = case.product(precision=precision)
+    lat.init_equilibrium(1.0)
+    lat.step(max(1, warmup))
+    lat.synchronize()
+    ms = time_steps(lat, steps)
+    lat.synchronize()
+    stable = lat.stable()
+    n = lat.n
+    lat.close()
+    return n, ms, stable
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,13 +209,8 @@ def main() -> int:
     log(f"warmup {args.warmup} steps done")
 
     # Timed region: HIP events recorded on the stream the kernels run on.
-    ms_total = 0.0
-    left = args.steps
     wall0 = time.perf_counter()
-    while left > 0:
-        chunk = min(left, 1024)
-        ms_total += lat.step_timed(chunk)
-        left -= chunk
+    ms_total = time_steps(lat, args.steps)
     wall = time.perf_counter() - wall0
     lat.synchronize()
     if not lat.stable():
@@ -199,6 +222,13 @@ def main() -> int:
     bytes_per_launch = n * BYTES_PER_SITE["f64"]
     achieved = bytes_per_launch / per_launch_s / 1e9
     log(f"{args.steps} steps in {ms_total:.2f} ms (host wall {wall*1e3:.2f} ms): {mflups:.1f} MFLUPS")
+    lat.close()
+
+    n32, ms32, stable32 = time_form(case, H.FP32_SHIFTED, args.steps, args.warmup)
+    mflups32 = n32 * args.steps / (ms32 / 1e3) / 1e6
+    achieved32 = n32 * BYTES_PER_SITE["f32"] / (ms32 / 1e3 / args.steps) / 1e9
+    log(f"fp32-shifted: {args.steps} steps in {ms32:.2f} ms: {mflups32:.1f} MFLUPS"
+        + ("" if stable32 else " (unstable)"))
 
     out = {
         "metric": "MFLUPS",
@@ -219,6 +249,14 @@ def main() -> int:
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": round(traffic["bytes_per_launch"]) if traffic else _committed_traffic(args.workload, n),
                      "bytes_per_site_update": BYTES_PER_SITE["f64"], "kernel": "step_fp64"},
+    }
+    out["forms"] = {
+        "fp64": {"mflups": round(mflups, 2), "ms_per_step": ms_total / args.steps,
+                 "bytes_per_site_update": BYTES_PER_SITE["f64"], "hbm_frac": round(achieved / HBM_PEAK_GBS, 4)},
+        "fp32_shifted": {"mflups": round(mflups32, 2), "ms_per_step": ms32 / args.steps,
+                         "bytes_per_site_update": BYTES_PER_SITE["f32"],
+                         "hbm_frac": round(achieved32 / HBM_PEAK_GBS, 4),
+                         "vs_fp64": round(mflups32 / mflups, 3), "kernel": "step_fp32"},
     }
     if not args.no_cpu_baseline:
         log("timing the CPU oracle on a bounded sample")

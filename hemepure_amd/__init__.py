@@ -109,6 +109,18 @@ _SIGS = {
     "hlb_timestep": (C.c_int64, [C.c_void_p]),
     "hlb_stability": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "hlb_iolet_state": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "hlb_batch_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "hlb_batch_destroy": (None, [C.c_void_p]),
+    "hlb_batch_add": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hlb_batch_remove": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hlb_batch_size": (C.c_int, [C.c_void_p]),
+    "hlb_batch_sites": (C.c_int64, [C.c_void_p]),
+    "hlb_batch_step": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint32]),
+    "hlb_batch_step_timed": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint32, C.POINTER(C.c_double)]),
+    "hlb_batch_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
+    "hlb_batch_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
+    "hlb_batch_variant": (C.c_int, [C.c_void_p]),
+    "hlb_batch_autotune": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "hlb_site_count": (C.c_int64, [C.c_void_p]),
     "hlb_class_counts": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hlb_site_coords": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -389,3 +401,65 @@ class Lattice:
         out = np.zeros(4, dtype=np.float64)
         _check(lib().hlb_iolet_state(self._h, int(bool(outlet)), int(index), _ptr(out)))
         return out
+
+
+class Batch:
+    """Many FP32-shifted lattices advanced by one step launch per time step
+    (hlb_batch_*).  Members stay owned by the caller."""
+
+    def __init__(self, members=()):
+        h = C.c_void_p()
+        _check(lib().hlb_batch_create(C.byref(h)))
+        self._h = h
+        self._members = []
+        for m in members:
+            self.add(m)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hlb_batch_destroy(self._h)
+            self._h = None
+            self._members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, lat: "Lattice"):
+        _check(lib().hlb_batch_add(self._h, lat._h))
+        self._members.append(lat)
+
+    def remove(self, lat: "Lattice"):
+        _check(lib().hlb_batch_remove(self._h, lat._h))
+        self._members = [m for m in self._members if m is not lat]
+
+    def size(self) -> int:
+        return int(lib().hlb_batch_size(self._h))
+
+    def sites(self) -> int:
+        return int(lib().hlb_batch_sites(self._h))
+
+    def step(self, nsteps: int, properties=False):
+        _check(lib().hlb_batch_step(self._h, nsteps, STEP_PROPERTIES if properties else 0))
+
+    def step_timed(self, nsteps: int, properties=False) -> float:
+        ms = C.c_double()
+        _check(lib().hlb_batch_step_timed(self._h, nsteps, STEP_PROPERTIES if properties else 0, C.byref(ms)))
+        return ms.value
+
+    def set_graphs(self, enable: bool):
+        _check(lib().hlb_batch_set_graphs(self._h, 1 if enable else 0))
+
+    def set_variant(self, variant: int):
+        _check(lib().hlb_batch_set_variant(self._h, variant))
+
+    def variant(self) -> int:
+        return int(lib().hlb_batch_variant(self._h))
+
+    def autotune(self, trial_steps=20):
+        v = C.c_int()
+        ms = C.c_double()
+        _check(lib().hlb_batch_autotune(self._h, trial_steps, C.byref(v), C.byref(ms)))
+        return v.value, ms.value

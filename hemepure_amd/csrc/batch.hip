@@ -1,0 +1,339 @@
+// batch.hip -- many FP32-shifted lattices advanced together.
+//
+// An ensemble (uncertainty-quantification members, serving requests) is a
+// set of independent lattices of uneven size.  A batch keeps each member's
+// arrays where hlb_create put them and advances all members with ONE step
+// launch per time step: the launch walks a member table (Member32, one record
+// per member with its plane pointers, tau and boundary selection) and a
+// tile -> member map, so members can join and leave between calls without
+// copying any lattice.  Windkessel iolets of all members are served by one
+// pair of per-step launches.  Sequences of steps are captured once into a
+// hipGraph per (member set, length, property flag) and replayed.
+//
+// Every member computes exactly what it computes alone: the per-site
+// arithmetic does not depend on the table, the tile size or the order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/hemepure_amd.h"
+#include "device_util.h"
+#include "errors.h"
+#include "handle.h"
+
+struct hlb_batch {
+  std::vector<hlb_lattice*> members;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, done = nullptr;
+  int device = -1;
+  int variant = hlb::kFp32DefaultVariant;
+  bool use_graph = true;
+  bool dirty = true;  // member set or variant changed: rebuild tables, drop graphs
+  uint32_t grid = 0;
+  hlb::DevBuf<hlb::Member32> mem;
+  hlb::DevBuf<int32_t> wg_member;
+  hlb::DevBuf<hlb::WkRef> refs;
+  int nwk = 0;
+  hlb::Member32* h_mem = nullptr;  // pinned staging of the member table
+  size_t h_mem_cap = 0;
+  struct Graph {
+    int64_t n;
+    int props_at;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
+
+  void drop_graphs() {
+    for (Graph& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
+  ~hlb_batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    drop_graphs();
+    for (hlb_lattice* m : members) m->batch = nullptr;
+    if (stream) (void)hipStreamDestroy(stream);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (done) (void)hipEventDestroy(done);
+    if (h_mem) (void)hipHostFree(h_mem);
+  }
+};
+
+namespace hlb {
+
+void batch_forget(hlb_batch* b, hlb_lattice* h) {
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  b->members.erase(std::remove(b->members.begin(), b->members.end(), h), b->members.end());
+  h->batch = nullptr;
+  b->dirty = true;
+}
+
+namespace {
+
+int ensure_stream(hlb_batch* b) {
+  if (b->stream) return HLB_OK;
+  HLB_CHECK(hipGetDevice(&b->device));
+  HLB_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  HLB_CHECK(hipEventCreate(&b->ev0));
+  HLB_CHECK(hipEventCreate(&b->ev1));
+  HLB_CHECK(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+  return HLB_OK;
+}
+
+// Tile -> member map for the current variant and the Windkessel references
+// of all members.
+int rebuild(hlb_batch* b) {
+  HLB_CHECK(hipStreamSynchronize(b->stream));
+  b->drop_graphs();
+  const int64_t B = fp32_variant_block(b->variant);
+  std::vector<int32_t> map;
+  std::vector<WkRef> refs;
+  for (size_t i = 0; i < b->members.size(); ++i) {
+    const hlb_lattice* m = b->members[i];
+    const int64_t tiles = (m->L.N + B - 1) / B;
+    map.insert(map.end(), (size_t)tiles, (int32_t)i);
+    refs.insert(refs.end(), m->wk_refs_host.begin(), m->wk_refs_host.end());
+  }
+  while (map.size() % kXcds != 0) map.push_back(-1);
+  b->grid = (uint32_t)map.size();
+  HLB_CHECK(b->wg_member.alloc(std::max<size_t>(1, map.size())));
+  if (!map.empty())
+    HLB_CHECK(hipMemcpy(b->wg_member.p, map.data(), sizeof(int32_t) * map.size(), hipMemcpyHostToDevice));
+  b->nwk = (int)refs.size();
+  HLB_CHECK(b->refs.alloc(std::max<size_t>(1, refs.size())));
+  if (!refs.empty()) HLB_CHECK(hipMemcpy(b->refs.p, refs.data(), sizeof(WkRef) * refs.size(), hipMemcpyHostToDevice));
+  const size_t need = std::max<size_t>(1, b->members.size());
+  if (b->h_mem_cap < need) {
+    if (b->h_mem) HLB_CHECK(hipHostFree(b->h_mem));
+    b->h_mem = nullptr;
+    HLB_CHECK(hipHostMalloc((void**)&b->h_mem, sizeof(Member32) * need, hipHostMallocDefault));
+    b->h_mem_cap = need;
+  }
+  HLB_CHECK(b->mem.alloc(need));
+  b->dirty = false;
+  return HLB_OK;
+}
+
+int enqueue_sequence(hlb_batch* b, int64_t n) {
+  for (int64_t s = 0; s < n; ++s) {
+    HLB_CHECK(launch_wk_begin(b->refs.p, b->nwk, (int)s, b->stream));
+    HLB_CHECK(launch_step_fp32(b->mem.p, b->wg_member.p, b->grid, (int)s, b->variant, b->stream));
+    HLB_CHECK(launch_wk_end(b->refs.p, b->nwk, b->stream));
+  }
+  return HLB_OK;
+}
+
+// Launches n steps whose iolet values and member records are staged, from
+// a captured graph when enabled.
+int launch_sequence(hlb_batch* b, int64_t n, int props_at) {
+  if (!b->use_graph) return enqueue_sequence(b, n);
+  for (const auto& g : b->graphs)
+    if (g.n == n && g.props_at == props_at) {
+      HLB_CHECK(hipGraphLaunch(g.exec, b->stream));
+      return HLB_OK;
+    }
+  hipGraph_t graph = nullptr;
+  HLB_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_sequence(b, n);
+  hipGraph_t captured = nullptr;
+  const hipError_t end = hipStreamEndCapture(b->stream, &captured);
+  if (rc != HLB_OK) {
+    if (captured) (void)hipGraphDestroy(captured);
+    return rc;
+  }
+  HLB_CHECK(end);
+  graph = captured;
+  hipGraphExec_t exec = nullptr;
+  const hipError_t inst = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HLB_CHECK(inst);
+  b->graphs.push_back({n, props_at, exec});
+  HLB_CHECK(hipGraphLaunch(exec, b->stream));
+  return HLB_OK;
+}
+
+int check_members(hlb_batch* b, const char* who) {
+  for (hlb_lattice* m : b->members) {
+    if (!m->f_valid) return fail(HLB_EINVAL, std::string(who) + ": a member has no initial condition");
+    HLB_CHECK(hipStreamSynchronize(m->stream));  // finish the member's own work first
+  }
+  return HLB_OK;
+}
+
+// Stages a chunk (<= kChunk steps): every member's iolet values and record.
+int stage_chunk(hlb_batch* b, int64_t n, int props_at) {
+  HLB_CHECK(hipStreamSynchronize(b->stream));  // staging buffers are free again
+  const int64_t B = fp32_variant_block(b->variant);
+  uint32_t wg = 0;
+  for (size_t i = 0; i < b->members.size(); ++i) {
+    hlb_lattice* m = b->members[i];
+    const int rc = stage_iolet_values(m, m->t, n, b->stream);
+    if (rc != HLB_OK) return rc;
+    Member32 r = member_record(m, props_at);
+    r.wg_begin = wg;
+    wg += (uint32_t)((m->L.N + B - 1) / B);
+    b->h_mem[i] = r;
+  }
+  HLB_CHECK(hipMemcpyAsync(b->mem.p, b->h_mem, sizeof(Member32) * b->members.size(), hipMemcpyHostToDevice,
+                           b->stream));
+  return HLB_OK;
+}
+
+// Launches a staged chunk and advances the members' host state.
+int launch_chunk(hlb_batch* b, int64_t n, int props_at) {
+  const int rc = launch_sequence(b, n, props_at);
+  if (rc != HLB_OK) return rc;
+  for (hlb_lattice* m : b->members) advance_state(m, n);
+  return HLB_OK;
+}
+
+// Later work on a member's own stream waits for the batch.
+int release_members(hlb_batch* b) {
+  HLB_CHECK(hipEventRecord(b->done, b->stream));
+  for (hlb_lattice* m : b->members) HLB_CHECK(hipStreamWaitEvent(m->stream, b->done, 0));
+  return HLB_OK;
+}
+
+int prepare(hlb_batch* b, const char* who) {
+  if (b->members.empty()) return HLB_OK;
+  int rc = ensure_stream(b);
+  if (rc != HLB_OK) return rc;
+  rc = check_members(b, who);
+  if (rc != HLB_OK) return rc;
+  if (b->dirty) rc = rebuild(b);
+  return rc;
+}
+
+}  // namespace
+}  // namespace hlb
+
+using namespace hlb;
+
+extern "C" int hlb_batch_create(hlb_batch_handle* out) {
+  if (!out) return fail(HLB_EINVAL, "hlb_batch_create: null argument");
+  *out = new (std::nothrow) hlb_batch();
+  return *out ? HLB_OK : fail(HLB_EINVAL, "hlb_batch_create: out of host memory");
+}
+
+extern "C" void hlb_batch_destroy(hlb_batch_handle b) { delete b; }
+
+extern "C" int hlb_batch_add(hlb_batch_handle b, hlb_handle h) {
+  if (!b || !h) return fail(HLB_EINVAL, "hlb_batch_add: null argument");
+  if (!h->fp32()) return fail(HLB_EINVAL, "hlb_batch_add: batches hold HLB_FP32_SHIFTED lattices");
+  if (h->batch) return fail(HLB_EINVAL, "hlb_batch_add: the lattice already belongs to a batch");
+  int rc = ensure_device(h);
+  if (rc != HLB_OK) return rc;
+  rc = ensure_stream(b);
+  if (rc != HLB_OK) return rc;
+  if (h->device != b->device) return fail(HLB_EINVAL, "hlb_batch_add: the lattice lives on another device");
+  if (b->stream) HLB_CHECK(hipStreamSynchronize(b->stream));
+  b->members.push_back(h);
+  h->batch = b;
+  b->dirty = true;
+  return HLB_OK;
+}
+
+extern "C" int hlb_batch_remove(hlb_batch_handle b, hlb_handle h) {
+  if (!b || !h) return fail(HLB_EINVAL, "hlb_batch_remove: null argument");
+  if (h->batch != b) return fail(HLB_EINVAL, "hlb_batch_remove: the lattice is not in this batch");
+  batch_forget(b, h);
+  return HLB_OK;
+}
+
+extern "C" int hlb_batch_size(hlb_batch_handle b) { return b ? (int)b->members.size() : -1; }
+
+extern "C" int64_t hlb_batch_sites(hlb_batch_handle b) {
+  if (!b) return -1;
+  int64_t n = 0;
+  for (const hlb_lattice* m : b->members) n += m->L.N;
+  return n;
+}
+
+extern "C" int hlb_batch_set_graphs(hlb_batch_handle b, int enable) {
+  if (!b) return fail(HLB_EINVAL, "hlb_batch_set_graphs: null batch");
+  if (b->stream) HLB_CHECK(hipStreamSynchronize(b->stream));
+  b->use_graph = enable != 0;
+  b->drop_graphs();
+  return HLB_OK;
+}
+
+extern "C" int hlb_batch_set_variant(hlb_batch_handle b, int variant) {
+  if (!b) return fail(HLB_EINVAL, "hlb_batch_set_variant: null batch");
+  if (variant < 0 || variant >= kFp32Variants) return fail(HLB_EINVAL, "hlb_batch_set_variant: no such variant");
+  if (b->stream) HLB_CHECK(hipStreamSynchronize(b->stream));
+  b->variant = variant;
+  b->dirty = true;
+  return HLB_OK;
+}
+
+extern "C" int hlb_batch_variant(hlb_batch_handle b) { return b ? b->variant : -1; }
+
+extern "C" int hlb_batch_step(hlb_batch_handle b, int64_t nsteps, uint32_t flags) {
+  if (!b) return fail(HLB_EINVAL, "hlb_batch_step: null batch");
+  if (nsteps < 0) return fail(HLB_EINVAL, "hlb_batch_step: negative step count");
+  int rc = prepare(b, "hlb_batch_step");
+  if (rc != HLB_OK || b->members.empty()) return rc;
+  for (int64_t done = 0; done < nsteps;) {
+    const int64_t n = std::min<int64_t>(kChunk, nsteps - done);
+    const int props_at = ((flags & HLB_STEP_PROPERTIES) && done + n == nsteps) ? (int)(n - 1) : -1;
+    rc = stage_chunk(b, n, props_at);
+    if (rc == HLB_OK) rc = launch_chunk(b, n, props_at);
+    if (rc != HLB_OK) return rc;
+    done += n;
+  }
+  return release_members(b);
+}
+
+extern "C" int hlb_batch_step_timed(hlb_batch_handle b, int64_t nsteps, uint32_t flags, double* ms) {
+  if (!b || !ms) return fail(HLB_EINVAL, "hlb_batch_step_timed: null argument");
+  if (nsteps < 0 || nsteps > kChunk) return fail(HLB_EINVAL, "hlb_batch_step_timed: 0 to 1024 steps per timed call");
+  int rc = prepare(b, "hlb_batch_step_timed");
+  if (rc != HLB_OK) return rc;
+  if (b->members.empty()) {
+    *ms = 0.0;
+    return HLB_OK;
+  }
+  // Stage first so the timed region holds only the launches.
+  const int props_at = (flags & HLB_STEP_PROPERTIES) ? (int)(nsteps - 1) : -1;
+  rc = stage_chunk(b, nsteps, props_at);
+  if (rc != HLB_OK) return rc;
+  HLB_CHECK(hipStreamSynchronize(b->stream));
+  HLB_CHECK(hipEventRecord(b->ev0, b->stream));
+  rc = launch_chunk(b, nsteps, props_at);
+  if (rc != HLB_OK) return rc;
+  HLB_CHECK(hipEventRecord(b->ev1, b->stream));
+  HLB_CHECK(hipEventSynchronize(b->ev1));
+  float e = 0.0f;
+  HLB_CHECK(hipEventElapsedTime(&e, b->ev0, b->ev1));
+  *ms = e;
+  return release_members(b);
+}
+
+// Times every launch shape on the live batch (the trial steps are real steps
+// of every member) and keeps the fastest.
+extern "C" int hlb_batch_autotune(hlb_batch_handle b, int64_t trial_steps, int* variant, double* ms_per_step) {
+  if (!b) return fail(HLB_EINVAL, "hlb_batch_autotune: null batch");
+  if (trial_steps < 1 || trial_steps > kChunk) return fail(HLB_EINVAL, "hlb_batch_autotune: 1 to 1024 trial steps");
+  int best = b->variant;
+  double best_ms = 1e300;
+  for (int v = 0; v < kFp32Variants; ++v) {
+    int rc = hlb_batch_set_variant(b, v);
+    if (rc != HLB_OK) return rc;
+    double ms = 0.0;
+    rc = hlb_batch_step_timed(b, trial_steps, 0, &ms);  // first call captures the graph
+    if (rc != HLB_OK) return rc;
+    rc = hlb_batch_step_timed(b, trial_steps, 0, &ms);
+    if (rc != HLB_OK) return rc;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = v;
+    }
+  }
+  const int rc = hlb_batch_set_variant(b, best);
+  if (rc != HLB_OK) return rc;
+  if (variant) *variant = best;
+  if (ms_per_step) *ms_per_step = best_ms / (double)trial_steps;
+  return HLB_OK;
+}

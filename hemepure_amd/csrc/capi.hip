@@ -22,6 +22,7 @@
 #include "../../include/hemepure_amd.h"
 #include "d3q19.h"
 #include "errors.h"
+#include "handle.h"
 #include "lattice.h"
 #include "lbm_kernels.h"
 
@@ -32,36 +33,10 @@ std::string& last_error() {
   return msg;
 }
 
-hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, hipStream_t s);
-hipError_t launch_gather_fp64(const StepArgs& a, double* F, int64_t S, hipStream_t s);
-hipError_t launch_fill_fp64(double* F, int64_t N, int64_t S, const double* vals, hipStream_t s);
-hipError_t launch_unstable_fp64(const double* F, int64_t N, int64_t S, int* flag, hipStream_t s);
-hipError_t launch_wk_begin(WkIolet* wk, int n, double* dens_row, hipStream_t s);
-hipError_t launch_wk_end(WkIolet* wk, int n, const double* contrib, const int32_t* list, hipStream_t s);
-hipError_t launch_step_fp32(const Member32* members, const int32_t* wg_member, uint32_t grid, int step,
-                            hipStream_t s);
-hipError_t launch_gather_fp32(const Member32* members, int64_t N, float* F, hipStream_t s);
-hipError_t launch_fill_fp32(float* F, int64_t N, int64_t S, const float* vals, hipStream_t s);
-hipError_t launch_unstable_fp32(const float* F, int64_t N, int64_t S, int* flag, hipStream_t s);
+}  // namespace hlb
 
+namespace hlb {
 namespace {
-
-constexpr double kPi = 3.14159265358979323846264338327950288;  // src/constants.h:17
-constexpr int64_t kChunk = 1024;  // steps of iolet values staged per upload
-
-int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
-
-struct HostIolet {
-  hlb_iolet_desc d;
-  double n[3];    // InOutLet::SetNormal: normalised in double (InOutLet.h:169-172)
-  float nf[3];    // the Vector3D<float> copy Nash uses (NashZerothOrderPressureDelegate.h:57-58)
-  std::vector<double> table;
-  double minSimDensity = 1.0;
-  int dirs0 = 0;               // direction closest to the normal (coupling direction)
-  int64_t centre = -1;         // centre site, reference order (BoundaryValues.cc:109-128)
-  bool centreCoupled = false;
-  bool IsWK() const { return d.kind == HLB_IOLET_WK2 || d.kind == HLB_IOLET_WK3; }
-};
 
 // The lattice direction closest to the iolet normal: the same std::sort over
 // |c_k/|c_k| - n|^2 as NashZerothOrderPressureDelegate.h:95-116.
@@ -94,98 +69,6 @@ double cosine_density(const HostIolet& io, uint64_t t0) {
   const double f = ((double)t0) / ((double)warm);
   return f * target + (1. - f) * io.minSimDensity;
 }
-
-template <typename T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-  hipError_t alloc(size_t count) {
-    if (p) {
-      (void)hipFree(p);
-      p = nullptr;
-    }
-    n = count;
-    if (count == 0) return hipSuccess;
-    return hipMalloc(&p, sizeof(T) * count);
-  }
-};
-
-}  // namespace
-}  // namespace hlb
-
-struct hlb_lattice {
-  hlb::HostLattice L;
-  hlb_params params{};
-  std::vector<hlb::HostIolet> iolets;  // inlets then outlets
-  int n_in = 0, n_out = 0;
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // Device order (natural block order): perm[d] = reference index, inv[k] = d.
-  std::vector<int32_t> perm, inv;
-  // Boundary sites (classes 1-5) and iolet sites in device order.
-  std::vector<int32_t> bsite;  // reference index of boundary slot b
-  std::vector<int32_t> isite;  // reference index of iolet slot i
-  int64_t S = 0, NB = 0, SB = 0, NI = 0, SI = 0;
-  hlb::DevBuf<double> A[2], F, G, ladd_s, rho, u, wss, io_n, io_dens, io_vmax, feq_vals;
-  hlb::DevBuf<uint32_t> pull, bwall, biolet;
-  hlb::DevBuf<int32_t> bidx, bgi, bii;
-  hlb::DevBuf<float> bq, bnrm, io_nf;
-  hlb::DevBuf<int> flag;
-  // FP32-shifted storage (params.precision == HLB_FP32_SHIFTED): float A, F,
-  // G and the one-member table the member-table kernel reads.
-  hlb::DevBuf<float> A32[2], F32, G32, feq32;
-  hlb::DevBuf<hlb::Member32> mem;
-  hlb::Member32* h_mem = nullptr;  // pinned staging of the member record
-  // Windkessel iolets: device state, coupling bit per iolet, per-site terms,
-  // coupling sites per iolet in the reference's site order.
-  bool any_wk = false;
-  std::vector<int32_t> wk_list;
-  std::vector<hlb::WkIolet> wk_host;
-  hlb::DevBuf<hlb::WkIolet> wk;
-  hlb::DevBuf<uint32_t> wk_bit;
-  hlb::DevBuf<double> wk_contrib;
-  hlb::DevBuf<int32_t> wk_list_d;
-  double* h_stage = nullptr;  // pinned staging for per-step iolet values
-  int cur = 0;                // A[cur] holds the latest post-collision values
-  bool have_post = false;     // false: the next step reads the pre-collision F
-  bool f_valid = false;
-  int64_t t = 1;              // 1-indexed step that runs next
-  double omega = 0, half_omega = 0, half_omega_minus = 0, les_k = 0;
-  std::vector<double> h_ladd_s;  // Ladd profile factors, 18 planes of SI, device iolet order
-  bool device_ready = false;
-
-  ~hlb_lattice() {
-    if (stream) {
-      (void)hipStreamSynchronize(stream);
-      (void)hipStreamDestroy(stream);
-    }
-    if (ev0) (void)hipEventDestroy(ev0);
-    if (ev1) (void)hipEventDestroy(ev1);
-    if (h_stage) (void)hipHostFree(h_stage);
-    if (h_mem) (void)hipHostFree(h_mem);
-  }
-  bool fp32() const { return params.precision == HLB_FP32_SHIFTED; }
-};
-
-namespace hlb {
-namespace {
-
-int dev_fail(hipError_t e, const char* what) {
-  return fail(HLB_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-#define HLB_CHECK(expr)                                 \
-  do {                                                  \
-    hipError_t _e = (expr);                             \
-    if (_e != hipSuccess) return dev_fail(_e, #expr);   \
-  } while (0)
 
 StepArgs make_args(hlb_lattice* h, const double* ain, double* aout) {
   StepArgs a{};
@@ -495,9 +378,26 @@ int upload_lattice(hlb_lattice* h) {
     if (!h->wk_list.empty())
       HLB_CHECK(hipMemcpy(h->wk_list_d.p, h->wk_list.data(), sizeof(int32_t) * h->wk_list.size(),
                           hipMemcpyHostToDevice));
+    h->wk_refs_host.clear();
+    for (size_t g = 0; g < h->iolets.size(); ++g)
+      if (h->iolets[g].IsWK())
+        h->wk_refs_host.push_back(WkRef{h->wk.p + g, h->io_dens.p, h->wk_contrib.p, h->wk_list_d.p, (int32_t)g, nio});
+    HLB_CHECK(h->wk_refs.alloc(std::max<size_t>(1, h->wk_refs_host.size())));
+    if (!h->wk_refs_host.empty())
+      HLB_CHECK(hipMemcpy(h->wk_refs.p, h->wk_refs_host.data(), sizeof(WkRef) * h->wk_refs_host.size(),
+                          hipMemcpyHostToDevice));
   }
   HLB_CHECK(hipStreamSynchronize(h->stream));
   return HLB_OK;
+}
+
+}  // namespace
+
+void advance_state(hlb_lattice* h, int64_t n) {
+  if (n <= 0) return;
+  h->cur = h->have_post ? (h->cur ^ (int)(n & 1)) : (h->cur ^ (int)((n - 1) & 1));
+  h->have_post = true;
+  h->t += n;
 }
 
 // Device state is created on first use so the host tables can be built and
@@ -513,6 +413,8 @@ int ensure_device(hlb_lattice* h) {
   h->device_ready = true;
   return HLB_OK;
 }
+
+namespace {
 
 int validate_params(const hlb_params& p, const hlb_iolet_desc* in, int nin, const hlb_iolet_desc* out, int nout) {
   if (p.kernel < HLB_KERNEL_LBGK || p.kernel > HLB_KERNEL_LES) return fail(HLB_EINVAL, "hlb_create: unknown kernel");
@@ -544,6 +446,8 @@ int validate_params(const hlb_params& p, const hlb_iolet_desc* in, int nin, cons
   }
   return HLB_OK;
 }
+
+}  // namespace
 
 // The member record of an FP32 handle for a launch sequence starting now.
 Member32 member_record(const hlb_lattice* h, int props_at) {
@@ -592,8 +496,11 @@ Member32 member_record(const hlb_lattice* h, int props_at) {
   return m;
 }
 
+namespace {
+
 uint32_t member_grid(int64_t N) {
-  const uint32_t tiles = (uint32_t)((N + 255) / 256);
+  const int64_t B = fp32_variant_block(kFp32DefaultVariant);
+  const uint32_t tiles = (uint32_t)((N + B - 1) / B);
   return (tiles + 7) / 8 * 8;
 }
 
@@ -606,7 +513,9 @@ int upload_member(hlb_lattice* h, int props_at) {
 }
 
 // Stages the per-step iolet values of steps t .. t+n-1 for the device.
-int stage_iolet_values(hlb_lattice* h, int64_t t, int64_t n) {
+}  // namespace
+
+int stage_iolet_values(hlb_lattice* h, int64_t t, int64_t n, hipStream_t stream) {
   const int nio = std::max<int>(1, (int)h->iolets.size());
   double* dens = h->h_stage;
   double* vmax = h->h_stage + (size_t)kChunk * nio;
@@ -614,31 +523,27 @@ int stage_iolet_values(hlb_lattice* h, int64_t t, int64_t n) {
     const int rc = iolet_values(h, t + s, dens + s * nio, vmax + s * nio);
     if (rc != HLB_OK) return rc;
   }
-  HLB_CHECK(hipMemcpyAsync(h->io_dens.p, dens, sizeof(double) * n * nio, hipMemcpyHostToDevice, h->stream));
-  HLB_CHECK(hipMemcpyAsync(h->io_vmax.p, vmax, sizeof(double) * n * nio, hipMemcpyHostToDevice, h->stream));
+  HLB_CHECK(hipMemcpyAsync(h->io_dens.p, dens, sizeof(double) * n * nio, hipMemcpyHostToDevice, stream));
+  HLB_CHECK(hipMemcpyAsync(h->io_vmax.p, vmax, sizeof(double) * n * nio, hipMemcpyHostToDevice, stream));
   return HLB_OK;
 }
+
+namespace {
 
 // Enqueues the launches of n steps whose iolet values are staged at rows 0..n-1.
 int enqueue_steps(hlb_lattice* h, int64_t n, bool props_last) {
   const int nio = std::max<int>(1, (int)h->iolets.size());
+  const int nwk = (int)h->wk_refs_host.size();
   if (h->fp32()) {
-    const bool first0 = !h->have_post;
     int rc = upload_member(h, props_last ? (int)(n - 1) : -1);
     if (rc != HLB_OK) return rc;
     const uint32_t grid = member_grid(h->L.N);
     for (int64_t s = 0; s < n; ++s) {
-      if (h->any_wk)
-        HLB_CHECK(launch_wk_begin(h->wk.p, (int)h->iolets.size(), h->io_dens.p + s * nio, h->stream));
-      HLB_CHECK(launch_step_fp32(h->mem.p, nullptr, grid, (int)s, h->stream));
-      if (h->any_wk)
-        HLB_CHECK(launch_wk_end(h->wk.p, (int)h->iolets.size(), h->wk_contrib.p, h->wk_list_d.p, h->stream));
+      HLB_CHECK(launch_wk_begin(h->wk_refs.p, nwk, (int)s, h->stream));
+      HLB_CHECK(launch_step_fp32(h->mem.p, nullptr, grid, (int)s, kFp32DefaultVariant, h->stream));
+      HLB_CHECK(launch_wk_end(h->wk_refs.p, nwk, h->stream));
     }
-    if (n > 0) {
-      h->cur = first0 ? (h->cur ^ (int)((n - 1) & 1)) : (h->cur ^ (int)(n & 1));
-      h->have_post = true;
-    }
-    h->t += n;
+    advance_state(h, n);
     return HLB_OK;
   }
   for (int64_t s = 0; s < n; ++s) {
@@ -649,11 +554,9 @@ int enqueue_steps(hlb_lattice* h, int64_t n, bool props_last) {
     StepArgs a = make_args(h, ain, aout);
     a.io_dens = h->io_dens.p + s * nio;
     a.io_vmax = h->io_vmax.p + s * nio;
-    if (h->any_wk)
-      HLB_CHECK(launch_wk_begin(h->wk.p, (int)h->iolets.size(), h->io_dens.p + s * nio, h->stream));
+    HLB_CHECK(launch_wk_begin(h->wk_refs.p, nwk, (int)s, h->stream));
     HLB_CHECK(launch_step_fp64(a, h->params.kernel, first, props, h->stream));
-    if (h->any_wk)
-      HLB_CHECK(launch_wk_end(h->wk.p, (int)h->iolets.size(), h->wk_contrib.p, h->wk_list_d.p, h->stream));
+    HLB_CHECK(launch_wk_end(h->wk_refs.p, nwk, h->stream));
     if (!first) h->cur ^= 1;
     h->have_post = true;
   }
@@ -670,7 +573,7 @@ int run_steps(hlb_lattice* h, int64_t nsteps, uint32_t flags) {
     // The staging buffers are reused per chunk: wait for the launches that
     // read the previous chunk's values.
     HLB_CHECK(hipStreamSynchronize(h->stream));
-    int rc = stage_iolet_values(h, h->t, chunk);
+    int rc = stage_iolet_values(h, h->t, chunk, h->stream);
     if (rc != HLB_OK) return rc;
     rc = enqueue_steps(h, chunk, (flags & HLB_STEP_PROPERTIES) && done + chunk == nsteps);
     if (rc != HLB_OK) return rc;
@@ -781,7 +684,10 @@ extern "C" int hlb_create(const hlb_geometry* geom, const hlb_params* params, co
   }
 }
 
-extern "C" void hlb_destroy(hlb_handle h) { delete h; }
+extern "C" void hlb_destroy(hlb_handle h) {
+  if (h && h->batch) batch_forget(h->batch, h);
+  delete h;
+}
 
 extern "C" int hlb_init_equilibrium(hlb_handle h, double rho0, int64_t t0) {
   if (!h) return fail(HLB_EINVAL, "hlb_init_equilibrium: null handle");
@@ -859,7 +765,7 @@ extern "C" int hlb_step_timed(hlb_handle h, int64_t nsteps, uint32_t flags, doub
   if (nsteps < 0 || nsteps > kChunk) return fail(HLB_EINVAL, "hlb_step_timed: 0 to 1024 steps per timed call");
   if (!h->f_valid) return fail(HLB_EINVAL, "hlb_step_timed: no initial condition");
   HLB_CHECK(hipStreamSynchronize(h->stream));
-  int rc = stage_iolet_values(h, h->t, nsteps);
+  int rc = stage_iolet_values(h, h->t, nsteps, h->stream);
   if (rc != HLB_OK) return rc;
   HLB_CHECK(hipEventRecord(h->ev0, h->stream));
   rc = enqueue_steps(h, nsteps, (flags & HLB_STEP_PROPERTIES) != 0);

@@ -1,0 +1,151 @@
+// handle.h -- the handle behind hlb_handle and the internals the single
+// lattice and batch entry points share.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hemepure_amd.h"
+#include "errors.h"
+#include "lattice.h"
+#include "lbm_kernels.h"
+
+namespace hlb {
+
+hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, hipStream_t s);
+hipError_t launch_gather_fp64(const StepArgs& a, double* F, int64_t S, hipStream_t s);
+hipError_t launch_fill_fp64(double* F, int64_t N, int64_t S, const double* vals, hipStream_t s);
+hipError_t launch_unstable_fp64(const double* F, int64_t N, int64_t S, int* flag, hipStream_t s);
+hipError_t launch_wk_begin(const WkRef* refs, int n, int step, hipStream_t s);
+hipError_t launch_wk_end(const WkRef* refs, int n, hipStream_t s);
+hipError_t launch_step_fp32(const Member32* members, const int32_t* wg_member, uint32_t grid, int step,
+                            int variant, hipStream_t s);
+hipError_t launch_gather_fp32(const Member32* members, int64_t N, float* F, hipStream_t s);
+hipError_t launch_fill_fp32(float* F, int64_t N, int64_t S, const float* vals, hipStream_t s);
+hipError_t launch_unstable_fp32(const float* F, int64_t N, int64_t S, int* flag, hipStream_t s);
+
+
+
+inline constexpr double kPi = 3.14159265358979323846264338327950288;  // src/constants.h:17
+inline constexpr int64_t kChunk = 1024;  // steps of iolet values staged per upload
+
+inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+struct HostIolet {
+  hlb_iolet_desc d;
+  double n[3];    // InOutLet::SetNormal: normalised in double (InOutLet.h:169-172)
+  float nf[3];    // the Vector3D<float> copy Nash uses (NashZerothOrderPressureDelegate.h:57-58)
+  std::vector<double> table;
+  double minSimDensity = 1.0;
+  int dirs0 = 0;               // direction closest to the normal (coupling direction)
+  int64_t centre = -1;         // centre site, reference order (BoundaryValues.cc:109-128)
+  bool centreCoupled = false;
+  bool IsWK() const { return d.kind == HLB_IOLET_WK2 || d.kind == HLB_IOLET_WK3; }
+};
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t count) {
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+    }
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc(&p, sizeof(T) * count);
+  }
+};
+
+inline int dev_fail(hipError_t e, const char* what) {
+  return fail(HLB_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HLB_CHECK(expr)                                 \
+  do {                                                  \
+    hipError_t _e = (expr);                             \
+    if (_e != hipSuccess) return dev_fail(_e, #expr);   \
+  } while (0)
+
+}  // namespace hlb
+
+struct hlb_lattice {
+  hlb::HostLattice L;
+  hlb_params params{};
+  std::vector<hlb::HostIolet> iolets;  // inlets then outlets
+  int n_in = 0, n_out = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // Device order (natural block order): perm[d] = reference index, inv[k] = d.
+  std::vector<int32_t> perm, inv;
+  // Boundary sites (classes 1-5) and iolet sites in device order.
+  std::vector<int32_t> bsite;  // reference index of boundary slot b
+  std::vector<int32_t> isite;  // reference index of iolet slot i
+  int64_t S = 0, NB = 0, SB = 0, NI = 0, SI = 0;
+  hlb::DevBuf<double> A[2], F, G, ladd_s, rho, u, wss, io_n, io_dens, io_vmax, feq_vals;
+  hlb::DevBuf<uint32_t> pull, bwall, biolet;
+  hlb::DevBuf<int32_t> bidx, bgi, bii;
+  hlb::DevBuf<float> bq, bnrm, io_nf;
+  hlb::DevBuf<int> flag;
+  // FP32-shifted storage (params.precision == HLB_FP32_SHIFTED): float A, F,
+  // G and the one-member table the member-table kernel reads.
+  hlb::DevBuf<float> A32[2], F32, G32, feq32;
+  hlb::DevBuf<hlb::Member32> mem;
+  hlb::Member32* h_mem = nullptr;  // pinned staging of the member record
+  // Windkessel iolets: device state, coupling bit per iolet, per-site terms,
+  // coupling sites per iolet in the reference's site order.
+  bool any_wk = false;
+  std::vector<int32_t> wk_list;
+  std::vector<hlb::WkIolet> wk_host;
+  hlb::DevBuf<hlb::WkIolet> wk;
+  hlb::DevBuf<uint32_t> wk_bit;
+  hlb::DevBuf<double> wk_contrib;
+  hlb::DevBuf<int32_t> wk_list_d;
+  std::vector<hlb::WkRef> wk_refs_host;  // one per Windkessel iolet
+  hlb::DevBuf<hlb::WkRef> wk_refs;
+  struct hlb_batch* batch = nullptr;     // the batch this lattice belongs to, if any
+  double* h_stage = nullptr;  // pinned staging for per-step iolet values
+  int cur = 0;                // A[cur] holds the latest post-collision values
+  bool have_post = false;     // false: the next step reads the pre-collision F
+  bool f_valid = false;
+  int64_t t = 1;              // 1-indexed step that runs next
+  double omega = 0, half_omega = 0, half_omega_minus = 0, les_k = 0;
+  std::vector<double> h_ladd_s;  // Ladd profile factors, 18 planes of SI, device iolet order
+  bool device_ready = false;
+
+  ~hlb_lattice() {
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (h_stage) (void)hipHostFree(h_stage);
+    if (h_mem) (void)hipHostFree(h_mem);
+  }
+  bool fp32() const { return params.precision == HLB_FP32_SHIFTED; }
+};
+
+namespace hlb {
+
+// Shared by the single-lattice and batch entry points (capi.hip).
+int ensure_device(hlb_lattice* h);
+int stage_iolet_values(hlb_lattice* h, int64_t t, int64_t n, hipStream_t s);
+Member32 member_record(const hlb_lattice* h, int props_at);
+// The lattice state after an n-step sequence (A parity, first-step flag, time).
+void advance_state(hlb_lattice* h, int64_t n);
+// Removes a lattice that is being destroyed from its batch (batch.hip).
+void batch_forget(struct hlb_batch* b, hlb_lattice* h);
+
+}  // namespace hlb

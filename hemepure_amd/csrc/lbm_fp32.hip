@@ -229,13 +229,15 @@ __device__ __forceinline__ int member_of_tile(const int32_t* wg_member, uint32_t
   return wg_member ? __builtin_amdgcn_readfirstlane(ldp(wg_member, tile)) : 0;
 }
 
-__global__ __launch_bounds__(kBlock) void step_fp32(const Member32* __restrict__ members,
-                                                    const int32_t* __restrict__ wg_member, int step) {
-  const uint32_t tile = xcd_tile();
+// B sites per workgroup tile; REMAP: XCD-contiguous tile order.
+template <int B, bool REMAP>
+__global__ __launch_bounds__(B) void step_fp32(const Member32* __restrict__ members,
+                                               const int32_t* __restrict__ wg_member, int step) {
+  const uint32_t tile = REMAP ? xcd_tile() : blockIdx.x;
   const int mi = member_of_tile(wg_member, tile);
   if (mi < 0) return;  // padding tile
   const Member32& m = members[mi];
-  const uint32_t d = (tile - m.wg_begin) * kBlock + threadIdx.x;
+  const uint32_t d = (tile - m.wg_begin) * B + threadIdx.x;
   if (d >= (uint32_t)m.N) return;
   const Planes P = member_planes(m, step);
   const uint32_t raw1 = ldp_nt(m.pull, d);
@@ -352,12 +354,24 @@ __global__ __launch_bounds__(kBlock) void unstable_fp32(const float* __restrict_
 
 }  // namespace
 
-// Grid of a member-table launch: `tiles` workgroup tiles padded to a multiple
-// of the XCD count (padding tiles map to member -1).
+int fp32_variant_block(int variant) { return variant / 2 == 0 ? 128 : (variant / 2 == 1 ? 256 : 512); }
+bool fp32_variant_remap(int variant) { return (variant & 1) != 0; }
+
+// `grid` workgroup tiles of the variant's size, padded to a multiple of the
+// XCD count (padding tiles map to member -1).
 hipError_t launch_step_fp32(const Member32* members, const int32_t* wg_member, uint32_t grid, int step,
-                            hipStream_t s) {
+                            int variant, hipStream_t s) {
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL(step_fp32, dim3(grid), dim3(kBlock), 0, s, members, wg_member, step);
+#define HLB_V(B, R) hipLaunchKernelGGL((step_fp32<B, R>), dim3(grid), dim3(B), 0, s, members, wg_member, step)
+  switch (variant) {
+    case 0: HLB_V(128, false); break;
+    case 1: HLB_V(128, true); break;
+    case 2: HLB_V(256, false); break;
+    case 3: HLB_V(256, true); break;
+    case 4: HLB_V(512, false); break;
+    default: HLB_V(512, true); break;
+  }
+#undef HLB_V
   return hipGetLastError();
 }
 

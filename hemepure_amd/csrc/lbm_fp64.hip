@@ -284,18 +284,18 @@ __global__ __launch_bounds__(kBlock) void unstable_fp64(const double* __restrict
 // Start of a step for the Windkessel iolets: InOutLetWK*::DoComms
 // (InOutLetWK3.cc:33-52) -- density <- densityNew, flowRate <- the previous
 // step's sum scaled by area / siteCount -- and the ghost density the step's
-// Nash links read.
-__global__ void wk_begin(WkIolet* __restrict__ wk, int n, double* __restrict__ dens_row) {
-  const int g = threadIdx.x;
-  if (g >= n) return;
-  WkIolet& w = wk[g];
-  if (w.kind != HLB_IOLET_WK2 && w.kind != HLB_IOLET_WK3) return;
+// Nash links read.  One thread per Windkessel iolet.
+__global__ __launch_bounds__(256) void wk_begin(const WkRef* __restrict__ refs, int n, int step) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const WkRef ref = refs[r];
+  WkIolet& w = *ref.w;
   w.density = w.densityNew;
   w.flowRate = w.flowRateNew;
   if (w.siteCount != 0) w.flowRate = w.flowRate * w.area / w.siteCount;
   w.flowRateNew = 0.0;
   w.siteCount = 0;
-  dens_row[g] = w.density;
+  ref.io_dens[(int64_t)step * ref.nio + ref.g] = w.density;
 }
 
 // End of a step: the coupling sites' terms summed in the reference's site
@@ -305,11 +305,12 @@ __global__ void wk_begin(WkIolet* __restrict__ wk, int n, double* __restrict__ d
 // reference's sequential flowRateNew += rounds exactly so).
 constexpr int kWkStage = 2048;
 
-__global__ __launch_bounds__(256) void wk_end(WkIolet* __restrict__ wk, const double* __restrict__ contrib,
-                                              const int32_t* __restrict__ list) {
+__global__ __launch_bounds__(256) void wk_end(const WkRef* __restrict__ refs) {
   __shared__ double stage[kWkStage];
-  WkIolet& w = wk[blockIdx.x];
-  if (w.kind != HLB_IOLET_WK2 && w.kind != HLB_IOLET_WK3) return;  // uniform per workgroup
+  const WkRef ref = refs[blockIdx.x];
+  WkIolet& w = *ref.w;
+  const double* contrib = ref.contrib;
+  const int32_t* list = ref.list;
   const int32_t lb = w.list_begin, le = w.list_end;
   double sum = w.flowRateNew;
   for (int32_t c0 = lb; c0 < le; c0 += kWkStage) {
@@ -345,14 +346,15 @@ __global__ __launch_bounds__(256) void wk_end(WkIolet* __restrict__ wk, const do
 
 }  // namespace
 
-hipError_t launch_wk_begin(WkIolet* wk, int n, double* dens_row, hipStream_t s) {
-  hipLaunchKernelGGL(wk_begin, dim3(1), dim3(kMaxIolets), 0, s, wk, n, dens_row);
+hipError_t launch_wk_begin(const WkRef* refs, int n, int step, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(wk_begin, dim3((n + 255) / 256), dim3(256), 0, s, refs, n, step);
   return hipGetLastError();
 }
 
-hipError_t launch_wk_end(WkIolet* wk, int n, const double* contrib, const int32_t* list, hipStream_t s) {
+hipError_t launch_wk_end(const WkRef* refs, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(wk_end, dim3(n), dim3(256), 0, s, wk, contrib, list);
+  hipLaunchKernelGGL(wk_end, dim3(n), dim3(256), 0, s, refs);
   return hipGetLastError();
 }
 

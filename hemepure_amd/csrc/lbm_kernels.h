@@ -32,7 +32,14 @@ namespace hlb {
 
 constexpr uint32_t kBoundaryFlag = 0x80000000u;
 constexpr uint32_t kIndexMask = 0x7fffffffu;
-constexpr int kMaxIolets = 256;  // one thread per iolet in the Windkessel kernels
+constexpr int kMaxIolets = 256;  // iolets per lattice
+
+// Launch shapes of the FP32 member-table kernel: workgroup tile of 128, 256
+// or 512 sites, each with and without the XCD-contiguous tile order.
+constexpr int kFp32Variants = 6;
+constexpr int kFp32DefaultVariant = 3;  // 256 sites, XCD order
+int fp32_variant_block(int variant);
+bool fp32_variant_remap(int variant);
 
 struct StepArgs {
   // Per-plane base pointers, passed as separate scalars so each access is one
@@ -116,6 +123,17 @@ struct WkIolet {
   int32_t kind;            // HLB_IOLET_WK2 / HLB_IOLET_WK3, anything else: not a Windkessel
   int32_t centre_coupled;  // the centre site is one of this iolet's coupling sites
   int32_t list_begin, list_end;  // coupling sites in the reference's site order
+};
+
+// A Windkessel iolet as the per-step kernels see it: its state and the
+// buffers of the lattice it belongs to.  Arrays of these let one launch
+// serve the Windkessel iolets of every lattice in a batch.
+struct WkRef {
+  WkIolet* w;
+  double* io_dens;        // the lattice's ghost-density rows (stride nio)
+  const double* contrib;  // the lattice's per-iolet-site u.(-n) terms
+  const int32_t* list;    // the lattice's coupling-site lists
+  int32_t g, nio;         // iolet index in its lattice, row stride
 };
 
 }  // namespace hlb

@@ -178,6 +178,38 @@ int64_t hlb_timestep(hlb_handle h);
  * distribution entering the next step is > 0 (and not NaN). */
 int hlb_stability(hlb_handle h, int* stable);
 
+/* ---- batches of FP32-shifted lattices ----------------------------------- */
+/* No counterpart in the reference, which runs one lattice per process: the
+ * ensemble and serving forms advance many independent lattices (members,
+ * each created with hlb_create and HLB_FP32_SHIFTED and given an initial
+ * condition) with one step launch per time step over all of them.  Members
+ * stay owned by the caller, keep their own step counter and iolet state, and
+ * compute exactly what they compute alone.  A member may join or leave
+ * between calls; destroying a member removes it.  hlb_step on a member and
+ * batch steps are ordered. */
+typedef struct hlb_batch* hlb_batch_handle;
+int hlb_batch_create(hlb_batch_handle* out);
+void hlb_batch_destroy(hlb_batch_handle b);
+int hlb_batch_add(hlb_batch_handle b, hlb_handle member);
+int hlb_batch_remove(hlb_batch_handle b, hlb_handle member);
+int hlb_batch_size(hlb_batch_handle b);
+int64_t hlb_batch_sites(hlb_batch_handle b);
+/* Advances every member by nsteps (HLB_STEP_PROPERTIES: fields of the last
+ * step).  Asynchronous with respect to the host. */
+int hlb_batch_step(hlb_batch_handle b, int64_t nsteps, uint32_t flags);
+/* Like hlb_batch_step for at most 1024 steps; blocks and returns the device
+ * milliseconds of the launches (HIP events on the batch's stream). */
+int hlb_batch_step_timed(hlb_batch_handle b, int64_t nsteps, uint32_t flags, double* ms);
+/* Step sequences are captured into HIP graphs and replayed (default on). */
+int hlb_batch_set_graphs(hlb_batch_handle b, int enable);
+/* Launch shape: 0..5 = workgroup tile of 128/256/512 sites, without/with the
+ * XCD-contiguous tile order. */
+int hlb_batch_set_variant(hlb_batch_handle b, int variant);
+int hlb_batch_variant(hlb_batch_handle b);
+/* Times each launch shape over trial_steps real steps of every member and
+ * keeps the fastest. */
+int hlb_batch_autotune(hlb_batch_handle b, int64_t trial_steps, int* variant, double* ms_per_step);
+
 /* Iolet state (outlet = 0: inlet `index`, 1: outlet `index`):
  * out[0] density used by the last step's ghost sites (for Windkessel iolets,
  * the committed density), out[1] densityNew, out[2] flowRate, out[3]

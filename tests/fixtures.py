@@ -179,6 +179,35 @@ def windkessel_cylinder_case(kind=H.WK3, radius=6.0, length=40, box_xy=16, tau=0
                                    resistance=resistance, r_c=r_c, r_p=r_p, capacitance=capacitance)])
 
 
+def stenosis_case(radius=8.0, length=60, reduction=0.5, tau=0.7, amp=5e-4, period=2000.0,
+                  kernel=H.KERNEL_TRT, wall_bc=H.WALL_BFL) -> Case:
+    """A stenosed pipe (Gaussian narrowing of the given area fraction, width
+    2R, at mid-length): BFL, TRT, cosine pressure inlet about 1 + amp, Nash
+    outlet at rho 1."""
+    g = H.Geometry.stenosed_pipe(radius, length, reduction, 2.0 * radius)
+    i0, o0 = g.inlets()[0], g.outlets()[0]
+    return Case(g, kernel=kernel, wall_bc=wall_bc, tau=tau,
+                inlets=[IoletSpec(H.COSINE, i0["position"], i0["normal"], density_mean=1.0 + amp,
+                                  density_amp=amp, period=period)],
+                outlets=[IoletSpec(H.COSINE, o0["position"], o0["normal"], density_mean=1.0)])
+
+
+def ensemble_cases(count: int, seed: int = 2026, r_range=(6.0, 24.0), l_range=(60, 400)):
+    """Members of the stenosed-pipe ensemble: radius U(r_range), length
+    U(l_range), area reduction U(0, 0.7), tau U(0.55, 0.9), inlet amplitude
+    U(1e-4, 1e-3), member i drawn in order from one seeded generator."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        r = float(rng.uniform(*r_range))
+        length = int(rng.uniform(*l_range))
+        red = float(rng.uniform(0.0, 0.7))
+        tau = float(rng.uniform(0.55, 0.9))
+        amp = float(rng.uniform(1e-4, 1e-3))
+        out.append(stenosis_case(r, length, red, tau, amp))
+    return out
+
+
 def sine_velocity_table(nsteps: int, peak: float, period: float) -> np.ndarray:
     """Peak inlet speed per 1-indexed step for a pulsatile inflow with the given
     peak and period, 0.5 peak (1 - cos(2 pi t / period)); the table layout of

@@ -75,3 +75,15 @@ def test_host_tables_without_device():
     assert lat.n == g.n_sites
     assert lat.timestep() == 1
     assert lat.class_counts().sum() == g.n_sites
+
+
+def test_batch_rejects_fp64_member_without_device():
+    g = _cyl()
+    lat = H.Lattice(g, H.params(), [H.iolet_desc(H.COSINE, (0, 0, 0), (0, 0, 1))],
+                    [H.iolet_desc(H.COSINE, (0, 0, 0), (0, 0, -1))])
+    b = H.Batch()
+    assert b.size() == 0 and b.sites() == 0
+    with pytest.raises(H.HlbError, match="FP32"):
+        b.add(lat)
+    with pytest.raises(H.HlbError, match="no such variant"):
+        b.set_variant(6)

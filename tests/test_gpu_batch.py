@@ -1,0 +1,140 @@
+"""Batches of FP32-shifted lattices (hlb_batch_*): each member of a batch
+must compute bit-for-bit what it computes alone, whatever else is in the
+batch, when it joined, the launch shape and whether the steps replay a
+captured graph."""
+import numpy as np
+import pytest
+
+import hemepure_amd as H
+from fixtures import (bifurcation_case, cylinder_case, ensemble_cases, stenosis_case,
+                      windkessel_cylinder_case)
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_cases():
+    return [
+        cylinder_case(radius=6.0, length=40, box_xy=16),
+        stenosis_case(radius=7.0, length=50, reduction=0.6, tau=0.62),
+        windkessel_cylinder_case(kind=H.WK3, length=24),
+        cylinder_case(radius=5.0, length=30, box_xy=16, tau=0.55, kernel=H.KERNEL_LES),
+        bifurcation_case(rp=8.0, lp=40.0, rd=6.5, ld=32.0, steps=400, peak=0.03, period=200.0),
+        stenosis_case(radius=9.0, length=70, reduction=0.3, tau=0.85, kernel=H.KERNEL_LBGK, wall_bc=H.WALL_SBB),
+    ]
+
+
+def make(case):
+    lat = case.product(precision=H.FP32_SHIFTED)
+    lat.init_equilibrium(1.0)
+    return lat
+
+
+def state(lat):
+    rho, u, wss = lat.fields()
+    return lat.download_f(), rho, u, wss
+
+
+def assert_same(a, b, tag):
+    for name, x, y in zip(("f", "rho", "u", "wss"), a, b):
+        assert np.array_equal(x, y), f"{tag}: {name} differs ({int((x != y).sum())} values)"
+
+
+def solo(case, steps):
+    lat = make(case)
+    lat.step(steps, properties=True)
+    return state(lat)
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_members_match_solo_runs(graphs):
+    cases = mixed_cases()
+    lats = [make(c) for c in cases]
+    b = H.Batch(lats)
+    b.set_graphs(graphs)
+    assert b.size() == len(cases) and b.sites() == sum(l.n for l in lats)
+    b.step(150)
+    b.step(150, properties=True)
+    for i, (c, l) in enumerate(zip(cases, lats)):
+        assert l.timestep() == 301
+        assert_same(state(l), solo(c, 300), f"member {i}")
+    wk = lats[2].iolet_state(True, 0)
+    ref = make(cases[2])
+    ref.step(300)
+    np.testing.assert_array_equal(wk, ref.iolet_state(True, 0))
+
+
+def test_graph_replays_with_eager_work_between():
+    """A captured sequence replayed several times, with eager solo steps and
+    synchronisations in between, matches the eager path every time."""
+    cases = mixed_cases()[:3]
+    a = [make(c) for c in cases]
+    e = [make(c) for c in cases]
+    ba, be = H.Batch(a), H.Batch(e)
+    be.set_graphs(False)
+    for rep in range(4):
+        ba.step(40)
+        be.step(40)
+        a[0].step(3)  # eager work on a member's own stream between replays
+        e[0].step(3)
+        a[0].synchronize()
+        for i in range(3):
+            assert np.array_equal(a[i].download_f(), e[i].download_f()), f"replay {rep} member {i}"
+
+
+def test_join_and_leave_between_calls():
+    c1, c2, c3 = stenosis_case(radius=6.0, length=40), cylinder_case(radius=7.0, length=36, box_xy=16), \
+        stenosis_case(radius=10.0, length=60, reduction=0.2, tau=0.7)
+    m1, m2, m3 = make(c1), make(c2), make(c3)
+    b = H.Batch([m1])
+    b.step(100)
+    b.add(m2)
+    b.step(120)
+    b.add(m3)
+    b.remove(m1)
+    b.step(80, properties=True)
+    m1.step(10, properties=True)  # a member that left continues alone
+    assert (m1.timestep(), m2.timestep(), m3.timestep()) == (231, 201, 81)
+    assert_same(state(m1), solo(c1, 230), "m1")
+    assert_same(state(m2), solo(c2, 200), "m2")
+    assert_same(state(m3), solo(c3, 80), "m3")
+
+
+def test_every_launch_shape_and_autotune():
+    cases = ensemble_cases(6, seed=11, r_range=(5.0, 9.0), l_range=(30, 70))
+    lats = [make(c) for c in cases]
+    b = H.Batch(lats)
+    total = 0
+    for v in range(6):
+        b.set_variant(v)
+        b.step(7)
+        total += 7
+    v, ms = b.autotune(5)
+    total += 6 * 2 * 5
+    assert 0 <= v < 6 and ms > 0 and b.variant() == v
+    b.step(9, properties=True)
+    total += 9
+    for i, (c, l) in enumerate(zip(cases, lats)):
+        assert_same(state(l), solo(c, total), f"member {i}")
+
+
+def test_destroying_a_member_removes_it():
+    cases = mixed_cases()[:2]
+    lats = [make(c) for c in cases]
+    b = H.Batch(lats)
+    b.step(5)
+    lats[0].close()
+    assert b.size() == 1
+    b.step(5, properties=True)
+    assert_same(state(lats[1]), solo(cases[1], 10), "survivor")
+
+
+def test_batch_rejects_fp64_and_double_membership():
+    c = cylinder_case(radius=5.0, length=20, box_xy=16)
+    b = H.Batch()
+    with pytest.raises(H.HlbError, match="FP32"):
+        b.add(c.product())
+    m = make(c)
+    b.add(m)
+    other = H.Batch()
+    with pytest.raises(H.HlbError, match="already belongs"):
+        other.add(m)
