@@ -175,6 +175,82 @@ def time_form(case, precision, steps: int, warmup: int):
     return n, ms, stable
 
 
+def percentile(xs, q):
+    import numpy as np
+
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q))
+
+
+def bench_ensemble(members: int, steps: int, solve_steps: int, workers: int):
+    """The stenosed-pipe ensemble: `members` pipes (r U(6,24), L U(60,400),
+    stenosis 0-70%, tau U(0.55,0.9), inlet amplitude U(1e-4,1e-3), seed 2026),
+    BFL + TRT.  Batched form: every member FP32-shifted in one batch (one
+    launch per step, graph-replayed, autotuned launch shape), timed over
+    `steps` steps.  Plain form: each member alone in FP64, one after
+    another, each timed over a short run.  A solve is `solve_steps` steps of
+    one member; every member arrives at t = 0, so the completion times are
+    all equal for the batch and cumulative for the plain loop."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import hemepure_amd as H
+    from fixtures import ensemble_cases
+
+    t0 = time.perf_counter()
+    cases = ensemble_cases(members, seed=2026, workers=workers)
+    with ThreadPoolExecutor(workers) as ex:
+        lats = list(ex.map(lambda c: c.product(precision=H.FP32_SHIFTED), cases))
+    sites = sum(l.n for l in lats)
+    log(f"ensemble: {members} members, {sites} sites, built in {time.perf_counter() - t0:.1f} s")
+    for l in lats:
+        l.init_equilibrium(1.0)
+    batch = H.Batch(lats)
+    variant, _ = batch.autotune(10)
+    batch.step_timed(min(steps, 1024))  # captures the timed sequence's graph
+    ms = 0.0
+    left = steps
+    while left > 0:
+        n = min(left, 1024)
+        ms += batch.step_timed(n)
+        left -= n
+    stable = all(l.stable() for l in lats)
+    ms_step = ms / steps
+    batch.close()
+    for l in lats:
+        l.close()
+    mflups_b = sites * steps / (ms / 1e3) / 1e6
+    t_solve_b = solve_steps * ms_step / 1e3
+    log(f"ensemble batched fp32: {ms_step:.3f} ms/step, {mflups_b:.1f} MFLUPS, variant {variant}"
+        + ("" if stable else " (unstable member)"))
+
+    with ThreadPoolExecutor(workers) as ex:
+        plain = list(ex.map(lambda c: c.product(precision=H.FP64), cases))
+    per_member = []
+    for l in plain:
+        l.init_equilibrium(1.0)
+        l.step(2)
+        per_member.append(l.step_timed(20) / 20)
+        l.close()
+    ms_plain = sum(per_member)
+    done = [solve_steps * x / 1e3 for x in per_member]
+    completion = [sum(done[:i + 1]) for i in range(len(done))]
+    mflups_p = sites / (ms_plain / 1e3) / 1e6
+    log(f"ensemble plain fp64 loop: {ms_plain:.3f} ms per step of all members, {mflups_p:.1f} MFLUPS")
+    return {
+        "workload": f"{members} stenosed pipes, r U(6,24) L U(60,400), stenosis 0-70% width 2R, tau "
+                    "U(0.55,0.9), inlet amplitude U(1e-4,1e-3), seed 2026; TRT lambda 3/16, BFL, Nash cosine",
+        "members": members, "sites": sites, "steps_timed": steps, "steps_per_solve": solve_steps,
+        "batched_fp32": {"mflups": round(mflups_b, 2), "ms_per_step": ms_step,
+                         "hbm_frac": round(sites * BYTES_PER_SITE["f32"] / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "solves_per_s": round(members / t_solve_b, 3), "p50_s": round(t_solve_b, 3),
+                         "p99_s": round(t_solve_b, 3), "launch_variant": variant, "hip_graph": True},
+        "plain_fp64_loop": {"mflups": round(mflups_p, 2), "ms_per_step_all_members": ms_plain,
+                            "solves_per_s": round(members / completion[-1], 3),
+                            "p50_s": round(percentile(completion, 50), 3),
+                            "p99_s": round(percentile(completion, 99), 3)},
+        "speedup_solves_per_s": round((members / t_solve_b) / (members / completion[-1]), 3),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +258,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="bifurcation", choices=["bifurcation", "cylinder"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--ensemble-members", type=int, default=128)
+    ap.add_argument("--ensemble-steps", type=int, default=200)
+    ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--calibrate", action="store_true", help="run the 2 GiB calibration copy and exit")
     ap.add_argument("--measure-traffic", action="store_true",
                     help="measure HBM bytes per launch with rocprofv3 counters before the timed run")
@@ -258,6 +338,8 @@ def main() -> int:
                          "hbm_frac": round(achieved32 / HBM_PEAK_GBS, 4),
                          "vs_fp64": round(mflups32 / mflups, 3), "kernel": "step_fp32"},
     }
+    if not args.no_ensemble:
+        out["ensemble"] = bench_ensemble(args.ensemble_members, args.ensemble_steps, 5000, args.workers)
     if not args.no_cpu_baseline:
         log("timing the CPU oracle on a bounded sample")
         out["cpu_baseline"] = cpu_baseline(case)

@@ -192,20 +192,27 @@ def stenosis_case(radius=8.0, length=60, reduction=0.5, tau=0.7, amp=5e-4, perio
                 outlets=[IoletSpec(H.COSINE, o0["position"], o0["normal"], density_mean=1.0)])
 
 
-def ensemble_cases(count: int, seed: int = 2026, r_range=(6.0, 24.0), l_range=(60, 400)):
+def ensemble_cases(count: int, seed: int = 2026, r_range=(6.0, 24.0), l_range=(60, 400), workers: int = 1):
     """Members of the stenosed-pipe ensemble: radius U(r_range), length
     U(l_range), area reduction U(0, 0.7), tau U(0.55, 0.9), inlet amplitude
-    U(1e-4, 1e-3), member i drawn in order from one seeded generator."""
+    U(1e-4, 1e-3), member i drawn in order from one seeded generator.  The
+    geometries are voxelised on `workers` threads (the native generator
+    releases the interpreter lock)."""
     rng = np.random.default_rng(seed)
-    out = []
+    draws = []
     for _ in range(count):
         r = float(rng.uniform(*r_range))
         length = int(rng.uniform(*l_range))
         red = float(rng.uniform(0.0, 0.7))
         tau = float(rng.uniform(0.55, 0.9))
         amp = float(rng.uniform(1e-4, 1e-3))
-        out.append(stenosis_case(r, length, red, tau, amp))
-    return out
+        draws.append((r, length, red, tau, amp))
+    if workers <= 1:
+        return [stenosis_case(*d) for d in draws]
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(lambda d: stenosis_case(*d), draws))
 
 
 def sine_velocity_table(nsteps: int, peak: float, period: float) -> np.ndarray:
