@@ -251,6 +251,84 @@ def bench_ensemble(members: int, steps: int, solve_steps: int, workers: int):
     }
 
 
+def bench_serving(count: int, rate: float, max_sites: float, workers: int, quantum: int = 500):
+    """The serving stream: `count` requests (pipes, stenoses, bifurcations,
+    20k..max_sites sites, LBGK or TRT) arriving as a Poisson process of
+    `rate` per second (seeded), each run to periodic steady state
+    (cycle-to-cycle max |du| < 1e-6, cosine period 2000, cap 40k steps) by the
+    request loop under a 200 GB admission budget, FP32-shifted.  Geometry is
+    voxelised before the clock starts (it is the client's input); table
+    building, admission, stepping, convergence tests and retirement are timed.
+    Plain: each request alone in FP64, one at a time in arrival order; its
+    service time is the request's step count times its FP64 ms/step measured
+    on the GPU, replayed against the same arrival times."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    import hemepure_amd as H
+    from fixtures import serving_requests
+
+    period, tol, cap = 2000, 1e-6, 40000
+    cases = serving_requests(count, seed=7, max_sites=max_sites, workers=workers)
+    rng = np.random.default_rng(8)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    sites = [c.geom.n_sites for c in cases]
+    log(f"serving: {count} requests, {sum(sites)} sites, arrivals over {arrivals[-1]:.1f} s")
+    srv = H.Server(budget_bytes=200 << 30, quantum_steps=quantum)
+    ids = [None] * count
+    nxt = 0
+    t0 = time.perf_counter()
+    finished = 0
+    while finished < count:
+        now = time.perf_counter() - t0
+        while nxt < count and arrivals[nxt] <= now:
+            ids[nxt] = cases[nxt].submit(srv, max_steps=cap, period=period, tolerance=tol)
+            nxt += 1
+        run, q = srv.advance()
+        if run == 0 and q == 0 and nxt < count:
+            time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
+        finished = sum(1 for i in ids if i is not None and srv.status(i).state >= H.REQ_CONVERGED)
+    makespan = time.perf_counter() - t0
+    st = [srv.status(i) for i in ids]
+    tts = [s.t_done - s.t_submit for s in st]
+    steps = [s.steps for s in st]
+    converged = sum(1 for s in st if s.state == H.REQ_CONVERGED)
+    site_steps = srv.site_steps()
+    srv.close()
+    log(f"serving: makespan {makespan:.1f} s, {converged}/{count} converged, p50 {percentile(tts, 50):.2f} s, "
+        f"p99 {percentile(tts, 99):.2f} s")
+
+    # Plain: FP64 alone, FIFO, from measured ms/step.
+    with ThreadPoolExecutor(workers) as ex:
+        plain = list(ex.map(lambda c: c.product(precision=H.FP64), cases))
+    service = []
+    for l, n in zip(plain, steps):
+        l.init_equilibrium(1.0)
+        l.step(2)
+        service.append(n * l.step_timed(20) / 20 / 1e3)
+        l.close()
+    done, t = [], 0.0
+    for a, s_ in zip(arrivals, service):
+        t = max(t, a) + s_
+        done.append(t - a)
+    plain_makespan = max(a + d for a, d in zip(arrivals, done))
+    return {
+        "workload": f"{count} requests (straight pipes, stenoses, Y-bifurcations; 2e4..{max_sites:.0e} sites "
+                    f"log-uniform; LBGK or TRT; BFL; cosine inlet period {period}), Poisson {rate}/s seed 8, "
+                    f"to cycle-to-cycle max|du| < {tol} (cap {cap} steps), budget 200 GB, quantum {quantum}",
+        "requests": count, "converged": converged, "site_steps": site_steps,
+        "served_fp32": {"mflups": round(site_steps / makespan / 1e6, 2), "makespan_s": round(makespan, 3),
+                        "solves_per_s": round(count / makespan, 4),
+                        "p50_s": round(percentile(tts, 50), 3), "p99_s": round(percentile(tts, 99), 3)},
+        "plain_fp64_fifo": {"mflups": round(sum(n * s for n, s in zip(sites, steps)) / plain_makespan / 1e6, 2),
+                            "busy_mflups": round(sum(n * s for n, s in zip(sites, steps)) / sum(service) / 1e6, 2),
+                            "makespan_s": round(plain_makespan, 3),
+                            "solves_per_s": round(count / plain_makespan, 4),
+                            "p50_s": round(percentile(done, 50), 3), "p99_s": round(percentile(done, 99), 3)},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,6 +340,10 @@ def main() -> int:
     ap.add_argument("--ensemble-members", type=int, default=128)
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream measurement")
+    ap.add_argument("--serving-requests", type=int, default=24)
+    ap.add_argument("--serving-rate", type=float, default=8.0, help="Poisson arrivals per second")
+    ap.add_argument("--serving-max-sites", type=float, default=5e6)
     ap.add_argument("--calibrate", action="store_true", help="run the 2 GiB calibration copy and exit")
     ap.add_argument("--measure-traffic", action="store_true",
                     help="measure HBM bytes per launch with rocprofv3 counters before the timed run")
@@ -340,6 +422,9 @@ def main() -> int:
     }
     if not args.no_ensemble:
         out["ensemble"] = bench_ensemble(args.ensemble_members, args.ensemble_steps, 5000, args.workers)
+    if not args.no_serving:
+        out["serving"] = bench_serving(args.serving_requests, args.serving_rate, args.serving_max_sites,
+                                       args.workers)
     if not args.no_cpu_baseline:
         log("timing the CPU oracle on a bounded sample")
         out["cpu_baseline"] = cpu_baseline(case)

@@ -92,6 +92,39 @@ class IoletGeom(C.Structure):
     ]
 
 
+class Request(C.Structure):
+    _fields_ = [
+        ("geom", C.POINTER(GeometryView)),
+        ("params", Params),
+        ("inlets", C.POINTER(IoletDesc)),
+        ("n_inlets", C.c_int32),
+        ("pad0_", C.c_int32),
+        ("outlets", C.POINTER(IoletDesc)),
+        ("n_outlets", C.c_int32),
+        ("pad1_", C.c_int32),
+        ("max_steps", C.c_int64),
+        ("period", C.c_int64),
+        ("tolerance", C.c_double),
+        ("rho0", C.c_double),
+    ]
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [
+        ("state", C.c_int32),
+        ("cycles", C.c_int32),
+        ("steps", C.c_int64),
+        ("sites", C.c_int64),
+        ("device_bytes", C.c_int64),
+        ("delta", C.c_double),
+        ("t_submit", C.c_double),
+        ("t_admit", C.c_double),
+        ("t_done", C.c_double),
+    ]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_CONVERGED, REQ_DONE, REQ_FAILED = 0, 1, 2, 3, 4
+
 _lib = None
 
 _SIGS = {
@@ -109,6 +142,16 @@ _SIGS = {
     "hlb_timestep": (C.c_int64, [C.c_void_p]),
     "hlb_stability": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "hlb_iolet_state": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "hlb_device_bytes": (C.c_int64, [C.c_void_p]),
+    "hlb_server_create": (C.c_int, [C.c_int64, C.c_int64, C.POINTER(C.c_void_p)]),
+    "hlb_server_destroy": (None, [C.c_void_p]),
+    "hlb_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_int64)]),
+    "hlb_server_advance": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "hlb_server_status": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(RequestStatus)]),
+    "hlb_server_fields": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hlb_server_release": (C.c_int, [C.c_void_p, C.c_int64]),
+    "hlb_server_site_steps": (C.c_int64, [C.c_void_p]),
+    "hlb_server_bytes_in_use": (C.c_int64, [C.c_void_p]),
     "hlb_batch_create": (C.c_int, [C.POINTER(C.c_void_p)]),
     "hlb_batch_destroy": (None, [C.c_void_p]),
     "hlb_batch_add": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -463,3 +506,66 @@ class Batch:
         ms = C.c_double()
         _check(lib().hlb_batch_autotune(self._h, trial_steps, C.byref(v), C.byref(ms)))
         return v.value, ms.value
+
+
+class Server:
+    """The solve-request loop (hlb_server_*): continuous batching of
+    FP32-shifted requests under a device-memory budget."""
+
+    def __init__(self, budget_bytes: int, quantum_steps: int = 100):
+        h = C.c_void_p()
+        _check(lib().hlb_server_create(int(budget_bytes), int(quantum_steps), C.byref(h)))
+        self._h = h
+        self.quantum = quantum_steps
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hlb_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, geom: Geometry, prm: Params, inlets=(), outlets=(), max_steps=1000, period=0,
+               tolerance=0.0, rho0=1.0) -> int:
+        view = geom.view()
+        ins, outs = _desc_array(inlets), _desc_array(outlets)
+        r = Request()
+        r.geom = C.pointer(view)
+        r.params = prm
+        r.inlets = C.cast(ins, C.POINTER(IoletDesc)) if ins is not None else None
+        r.n_inlets = len(inlets)
+        r.outlets = C.cast(outs, C.POINTER(IoletDesc)) if outs is not None else None
+        r.n_outlets = len(outlets)
+        r.max_steps, r.period, r.tolerance, r.rho0 = int(max_steps), int(period), float(tolerance), float(rho0)
+        rid = C.c_int64()
+        _check(lib().hlb_server_submit(self._h, C.byref(r), C.byref(rid)))
+        return rid.value
+
+    def advance(self):
+        run, q = C.c_int(), C.c_int()
+        _check(lib().hlb_server_advance(self._h, C.byref(run), C.byref(q)))
+        return run.value, q.value
+
+    def status(self, rid: int) -> RequestStatus:
+        st = RequestStatus()
+        _check(lib().hlb_server_status(self._h, rid, C.byref(st)))
+        return st
+
+    def fields(self, rid: int):
+        n = self.status(rid).sites
+        rho, u, wss = np.zeros(n), np.zeros((n, 3)), np.zeros(n)
+        _check(lib().hlb_server_fields(self._h, rid, _ptr(rho), _ptr(u), _ptr(wss)))
+        return rho, u, wss
+
+    def release(self, rid: int):
+        _check(lib().hlb_server_release(self._h, rid))
+
+    def site_steps(self) -> int:
+        return int(lib().hlb_server_site_steps(self._h))
+
+    def bytes_in_use(self) -> int:
+        return int(lib().hlb_server_bytes_in_use(self._h))

@@ -19,7 +19,7 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libhemepure_amd.so"
 
-SOURCES = ["capi.hip", "batch.hip", "lbm_fp64.hip", "lbm_fp32.hip", "lattice.cpp", "geometry_gen.cpp"]
+SOURCES = ["capi.hip", "batch.hip", "server.hip", "lbm_fp64.hip", "lbm_fp32.hip", "lattice.cpp", "geometry_gen.cpp"]
 ARCH = os.environ.get("HLB_OFFLOAD_ARCH", "gfx950")
 
 

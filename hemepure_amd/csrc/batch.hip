@@ -63,6 +63,8 @@ struct hlb_batch {
 
 namespace hlb {
 
+hipStream_t batch_stream(hlb_batch* b) { return b->stream; }
+
 void batch_forget(hlb_batch* b, hlb_lattice* h) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   b->members.erase(std::remove(b->members.begin(), b->members.end(), h), b->members.end());

@@ -903,6 +903,24 @@ extern "C" int hlb_wall_geometry(hlb_handle h, double* dist, double* normal) {
   return HLB_OK;
 }
 
+namespace hlb {
+
+int64_t device_bytes(const hlb_lattice* h) {
+  const int64_t S = h->S, SB = h->SB, SI = h->SI;
+  const int64_t fsz = h->fp32() ? 4 : 8;
+  int64_t b = 3 * Q * fsz * S + Q * fsz * SI;        // A[2], F, G
+  b += 18 * 4 * S + 4 * S;                            // pull, bidx
+  b += (4 * 4 + 18 * 4 + 3 * 4) * SB;                 // boundary masks, ids, q, normals
+  b += 18 * 8 * SI + 8 * SI;                          // Ladd factors, Windkessel terms
+  b += 5 * 8 * S;                                     // rho, u, wss
+  b += (int64_t)(2 * kChunk + 8) * 8 * std::max<int64_t>(1, (int64_t)h->iolets.size());  // iolet rows
+  return b;
+}
+
+}  // namespace hlb
+
+extern "C" int64_t hlb_device_bytes(hlb_handle h) { return h ? hlb::device_bytes(h) : -1; }
+
 extern "C" int hlb_iolet_state(hlb_handle h, int outlet, int index, double out[4]) {
   if (!h || !out) return fail(HLB_EINVAL, "hlb_iolet_state: null argument");
   const int n = outlet ? h->n_out : h->n_in;

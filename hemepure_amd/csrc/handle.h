@@ -26,6 +26,8 @@ hipError_t launch_step_fp32(const Member32* members, const int32_t* wg_member, u
 hipError_t launch_gather_fp32(const Member32* members, int64_t N, float* F, hipStream_t s);
 hipError_t launch_fill_fp32(float* F, int64_t N, int64_t S, const float* vals, hipStream_t s);
 hipError_t launch_unstable_fp32(const float* F, int64_t N, int64_t S, int* flag, hipStream_t s);
+hipError_t launch_cycle_delta(const double* u, float* prev, int64_t N, int64_t S, unsigned* out_bits,
+                              hipStream_t s);
 
 
 
@@ -147,5 +149,9 @@ Member32 member_record(const hlb_lattice* h, int props_at);
 void advance_state(hlb_lattice* h, int64_t n);
 // Removes a lattice that is being destroyed from its batch (batch.hip).
 void batch_forget(struct hlb_batch* b, hlb_lattice* h);
+// Device bytes a lattice holds once it has a state (capi.hip).
+int64_t device_bytes(const hlb_lattice* h);
+// The stream a batch's launches run on (batch.hip).
+hipStream_t batch_stream(struct hlb_batch* b);
 
 }  // namespace hlb

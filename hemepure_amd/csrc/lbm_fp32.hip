@@ -352,7 +352,39 @@ __global__ __launch_bounds__(kBlock) void unstable_fp32(const float* __restrict_
   if (bad) *flag = 1;
 }
 
+// Cycle-to-cycle change of the velocity field: max over sites and components
+// of |u - prev| (prev is the float snapshot of the previous cycle), then
+// prev <- u.  The maximum is combined per wavefront and then across waves with
+// an unsigned atomic max on the float bits (exact for non-negative floats).
+// A NaN velocity reports +inf.
+__global__ __launch_bounds__(kBlock) void cycle_delta(const double* __restrict__ u, float* __restrict__ prev,
+                                                      int64_t N, int64_t S, unsigned* __restrict__ out_bits) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  float d = 0.0f;
+  if (k < N) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double v = u[(int64_t)a * S + k];
+      const float pv = prev[(int64_t)a * S + k];
+      const double e = __builtin_fabs(v - (double)pv);
+      d = (e == e) ? __builtin_fmaxf(d, (float)e) : __builtin_inff();
+      prev[(int64_t)a * S + k] = (float)v;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) d = __builtin_fmaxf(d, __shfl_xor(d, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out_bits, __float_as_uint(d));
+}
+
 }  // namespace
+
+hipError_t launch_cycle_delta(const double* u, float* prev, int64_t N, int64_t S, unsigned* out_bits,
+                              hipStream_t s) {
+  if (N == 0) return hipSuccess;
+  hipLaunchKernelGGL(cycle_delta, dim3((unsigned)((N + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, u, prev, N, S,
+                     out_bits);
+  return hipGetLastError();
+}
 
 int fp32_variant_block(int variant) { return variant / 2 == 0 ? 128 : (variant / 2 == 1 ? 256 : 512); }
 bool fp32_variant_remap(int variant) { return (variant & 1) != 0; }

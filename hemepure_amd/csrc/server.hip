@@ -1,0 +1,281 @@
+// server.hip -- a solve-request loop over one batch (continuous batching).
+//
+// Requests (geometry, parameters, step cap, convergence tolerance) are
+// submitted at any time; the lattice tables are built on the host at
+// submission.  Each call of hlb_server_advance
+//   1. admits queued requests in arrival order while their device bytes fit
+//      the budget (initial equilibrium, then join the batch),
+//   2. advances every running request by one quantum in the batch (one launch
+//      per step for all of them, the quantum's last step fills u),
+//   3. for requests at the end of a cycle, measures the cycle-to-cycle
+//      max |du| on the device,
+//   4. retires requests that converged or reached their cap: their rho, u and
+//      WSS are copied to host memory and their device memory freed.
+// A request computes exactly what the same lattice computes alone with
+// hlb_step (the batch guarantees it), so its result does not depend on what
+// else was running.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/hemepure_amd.h"
+#include "errors.h"
+#include "handle.h"
+
+namespace hlb {
+namespace {
+
+struct Request {
+  int64_t id = 0;
+  hlb_handle lat = nullptr;
+  int64_t bytes = 0;
+  int64_t max_steps = 0;
+  int64_t period = 0;
+  double tol = 0.0;
+  double rho0 = 1.0;
+  int state = HLB_REQ_QUEUED;
+  int64_t steps = 0;
+  int cycles = 0;
+  double delta = INFINITY;
+  double t_submit = 0, t_admit = -1, t_done = -1;
+  float* prev = nullptr;  // device snapshot of u at the last cycle end (3 planes of S)
+  std::vector<double> rho, u, wss;
+  std::string error;
+};
+
+}  // namespace
+}  // namespace hlb
+
+struct hlb_server {
+  int64_t budget = 0, in_use = 0, quantum = 0;
+  hlb_batch_handle batch = nullptr;
+  std::map<int64_t, std::unique_ptr<hlb::Request>> reqs;
+  std::deque<int64_t> queue;
+  std::vector<int64_t> running;
+  int64_t next_id = 1;
+  int64_t site_steps = 0;  // sites x steps advanced, all requests
+  unsigned* d_delta = nullptr;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  ~hlb_server() {
+    for (auto& kv : reqs) {
+      if (kv.second->lat) hlb_destroy(kv.second->lat);
+      if (kv.second->prev) (void)hipFree(kv.second->prev);
+    }
+    if (batch) hlb_batch_destroy(batch);
+    if (d_delta) (void)hipFree(d_delta);
+  }
+};
+
+namespace hlb {
+namespace {
+
+void release_device(hlb_server* s, Request* r) {
+  if (r->lat) {
+    hlb_destroy(r->lat);  // leaves the batch
+    r->lat = nullptr;
+  }
+  if (r->prev) {
+    (void)hipFree(r->prev);
+    r->prev = nullptr;
+  }
+  s->in_use -= r->bytes;
+}
+
+int admit(hlb_server* s) {
+  while (!s->queue.empty()) {
+    Request* r = s->reqs[s->queue.front()].get();
+    if (s->in_use + r->bytes > s->budget) break;  // arrival order: wait for memory
+    s->queue.pop_front();
+    int rc = hlb_init_equilibrium(r->lat, r->rho0, 1);
+    if (rc == HLB_OK) {
+      const int64_t S = r->lat->S;
+      const hipError_t e = hipMalloc(&r->prev, sizeof(float) * 3 * S);
+      rc = e == hipSuccess ? HLB_OK : dev_fail(e, "hipMalloc(snapshot)");
+    }
+    if (rc == HLB_OK) rc = hlb_batch_add(s->batch, r->lat);
+    s->in_use += r->bytes;
+    if (rc != HLB_OK) {
+      r->error = last_error();
+      r->state = HLB_REQ_FAILED;
+      r->t_done = s->now();
+      release_device(s, r);
+      continue;
+    }
+    r->state = HLB_REQ_RUNNING;
+    r->t_admit = s->now();
+    s->running.push_back(r->id);
+  }
+  return HLB_OK;
+}
+
+int retire(hlb_server* s, Request* r, int state) {
+  const int64_t n = r->lat->L.N;
+  r->rho.resize(n);
+  r->u.resize(3 * n);
+  r->wss.resize(n);
+  const int rc = hlb_get_fields(r->lat, r->rho.data(), r->u.data(), r->wss.data());
+  if (rc != HLB_OK) return rc;
+  r->state = state;
+  r->t_done = s->now();
+  release_device(s, r);
+  return HLB_OK;
+}
+
+}  // namespace
+}  // namespace hlb
+
+using namespace hlb;
+
+extern "C" int hlb_server_create(int64_t budget_bytes, int64_t quantum_steps, hlb_server_handle* out) {
+  if (!out) return fail(HLB_EINVAL, "hlb_server_create: null argument");
+  if (budget_bytes <= 0 || quantum_steps < 1 || quantum_steps > kChunk)
+    return fail(HLB_EINVAL, "hlb_server_create: budget must be positive and the quantum 1 to 1024 steps");
+  auto s = std::make_unique<hlb_server>();
+  s->budget = budget_bytes;
+  s->quantum = quantum_steps;
+  int rc = hlb_batch_create(&s->batch);
+  if (rc != HLB_OK) return rc;
+  *out = s.release();
+  return HLB_OK;
+}
+
+extern "C" void hlb_server_destroy(hlb_server_handle s) { delete s; }
+
+extern "C" int hlb_server_submit(hlb_server_handle s, const hlb_request* q, int64_t* id) {
+  if (!s || !q || !id || !q->geom) return fail(HLB_EINVAL, "hlb_server_submit: null argument");
+  if (q->params.precision != HLB_FP32_SHIFTED)
+    return fail(HLB_EINVAL, "hlb_server_submit: requests run in the HLB_FP32_SHIFTED form");
+  if (q->max_steps < 1) return fail(HLB_EINVAL, "hlb_server_submit: max_steps must be positive");
+  if (q->tolerance > 0.0 && (q->period < 1 || q->period % s->quantum != 0))
+    return fail(HLB_EINVAL, "hlb_server_submit: the convergence period must be a multiple of the quantum");
+  auto r = std::make_unique<Request>();
+  hlb_handle h = nullptr;
+  int rc = hlb_create(q->geom, &q->params, q->inlets, q->n_inlets, q->outlets, q->n_outlets, &h);
+  if (rc != HLB_OK) return rc;
+  r->id = s->next_id++;
+  r->lat = h;
+  r->bytes = device_bytes(h) + (int64_t)sizeof(float) * 3 * h->S;
+  r->max_steps = q->max_steps;
+  r->period = q->period;
+  r->tol = q->tolerance;
+  r->rho0 = q->rho0 > 0.0 ? q->rho0 : 1.0;
+  r->t_submit = s->now();
+  if (r->bytes > s->budget) {
+    hlb_destroy(h);
+    return fail(HLB_ERANGE, "hlb_server_submit: the request alone needs " + std::to_string(r->bytes) +
+                                " device bytes, more than the budget");
+  }
+  *id = r->id;
+  s->queue.push_back(r->id);
+  s->reqs[r->id] = std::move(r);
+  return HLB_OK;
+}
+
+extern "C" int hlb_server_advance(hlb_server_handle s, int* running, int* queued) {
+  if (!s) return fail(HLB_EINVAL, "hlb_server_advance: null server");
+  int rc = admit(s);
+  if (rc != HLB_OK) return rc;
+  if (!s->running.empty()) {
+    // Every running request advances by the quantum, capped per request by
+    // retiring it as soon as it reaches its cap (caps are rounded up to the
+    // quantum).
+    rc = hlb_batch_step(s->batch, s->quantum, HLB_STEP_PROPERTIES);
+    if (rc != HLB_OK) return rc;
+    hipStream_t st = batch_stream(s->batch);
+    if (!s->d_delta) HLB_CHECK(hipMalloc(&s->d_delta, sizeof(unsigned)));
+    std::vector<Request*> at_cycle;
+    for (int64_t id : s->running) {
+      Request* r = s->reqs[id].get();
+      r->steps += s->quantum;
+      s->site_steps += s->quantum * r->lat->L.N;
+      if (r->tol > 0.0 && r->steps % r->period == 0) at_cycle.push_back(r);
+    }
+    // Cycle ends: one reduction per request, read back together.
+    std::vector<unsigned> bits(at_cycle.size(), 0u);
+    for (size_t i = 0; i < at_cycle.size(); ++i) {
+      Request* r = at_cycle[i];
+      HLB_CHECK(hipMemsetAsync(s->d_delta, 0, sizeof(unsigned), st));
+      HLB_CHECK(launch_cycle_delta(r->lat->u.p, r->prev, r->lat->L.N, r->lat->S, s->d_delta, st));
+      HLB_CHECK(hipMemcpyAsync(&bits[i], s->d_delta, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    }
+    HLB_CHECK(hipStreamSynchronize(st));
+    for (size_t i = 0; i < at_cycle.size(); ++i) {
+      Request* r = at_cycle[i];
+      float d;
+      std::memcpy(&d, &bits[i], sizeof d);
+      r->cycles += 1;
+      r->delta = r->cycles >= 2 ? (double)d : INFINITY;  // the first cycle end only takes the snapshot
+    }
+    std::vector<int64_t> still;
+    for (int64_t id : s->running) {
+      Request* r = s->reqs[id].get();
+      const bool converged = r->tol > 0.0 && r->delta < r->tol;
+      if (converged || r->steps >= r->max_steps) {
+        rc = retire(s, r, converged ? HLB_REQ_CONVERGED : HLB_REQ_DONE);
+        if (rc != HLB_OK) return rc;
+      } else {
+        still.push_back(id);
+      }
+    }
+    s->running.swap(still);
+    rc = admit(s);  // memory freed by retirements
+    if (rc != HLB_OK) return rc;
+  }
+  if (running) *running = (int)s->running.size();
+  if (queued) *queued = (int)s->queue.size();
+  return HLB_OK;
+}
+
+extern "C" int hlb_server_status(hlb_server_handle s, int64_t id, hlb_request_status* out) {
+  if (!s || !out) return fail(HLB_EINVAL, "hlb_server_status: null argument");
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return fail(HLB_EINVAL, "hlb_server_status: no such request");
+  const Request* r = it->second.get();
+  out->state = r->state;
+  out->cycles = r->cycles;
+  out->steps = r->steps;
+  out->sites = r->lat ? r->lat->L.N : (int64_t)r->rho.size();
+  out->device_bytes = r->bytes;
+  out->delta = r->delta;
+  out->t_submit = r->t_submit;
+  out->t_admit = r->t_admit;
+  out->t_done = r->t_done;
+  return HLB_OK;
+}
+
+extern "C" int hlb_server_fields(hlb_server_handle s, int64_t id, double* rho, double* u, double* wss) {
+  if (!s) return fail(HLB_EINVAL, "hlb_server_fields: null server");
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return fail(HLB_EINVAL, "hlb_server_fields: no such request");
+  const Request* r = it->second.get();
+  if (r->state != HLB_REQ_CONVERGED && r->state != HLB_REQ_DONE)
+    return fail(HLB_EINVAL, "hlb_server_fields: the request has not finished");
+  if (rho) std::memcpy(rho, r->rho.data(), sizeof(double) * r->rho.size());
+  if (u) std::memcpy(u, r->u.data(), sizeof(double) * r->u.size());
+  if (wss) std::memcpy(wss, r->wss.data(), sizeof(double) * r->wss.size());
+  return HLB_OK;
+}
+
+extern "C" int hlb_server_release(hlb_server_handle s, int64_t id) {
+  if (!s) return fail(HLB_EINVAL, "hlb_server_release: null server");
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return fail(HLB_EINVAL, "hlb_server_release: no such request");
+  if (it->second->state == HLB_REQ_RUNNING || it->second->state == HLB_REQ_QUEUED)
+    return fail(HLB_EINVAL, "hlb_server_release: the request has not finished");
+  s->reqs.erase(it);
+  return HLB_OK;
+}
+
+extern "C" int64_t hlb_server_site_steps(hlb_server_handle s) { return s ? s->site_steps : -1; }
+extern "C" int64_t hlb_server_bytes_in_use(hlb_server_handle s) { return s ? s->in_use : -1; }

@@ -210,6 +210,60 @@ int hlb_batch_variant(hlb_batch_handle b);
  * keeps the fastest. */
 int hlb_batch_autotune(hlb_batch_handle b, int64_t trial_steps, int* variant, double* ms_per_step);
 
+/* ---- serving: a solve-request loop over one batch ------------------------ */
+/* No counterpart in the reference.  Requests are submitted at any time and
+ * run in the FP32-shifted form; each hlb_server_advance admits queued
+ * requests in arrival order while their device bytes fit the budget, advances
+ * all running requests by one quantum in one batch, measures the
+ * cycle-to-cycle max |du| of requests at a cycle end, and retires requests
+ * that converged (max |du| < tolerance) or reached max_steps, keeping their
+ * rho, u and WSS on the host.  Times are seconds since hlb_server_create. */
+#define HLB_REQ_QUEUED 0
+#define HLB_REQ_RUNNING 1
+#define HLB_REQ_CONVERGED 2
+#define HLB_REQ_DONE 3   /* reached max_steps without converging */
+#define HLB_REQ_FAILED 4
+
+typedef struct hlb_request {
+  const hlb_geometry* geom;
+  hlb_params params;            /* precision must be HLB_FP32_SHIFTED */
+  const hlb_iolet_desc* inlets;
+  int32_t n_inlets;
+  int32_t pad0_;
+  const hlb_iolet_desc* outlets;
+  int32_t n_outlets;
+  int32_t pad1_;
+  int64_t max_steps;            /* cap, rounded up to the quantum */
+  int64_t period;               /* steps per cycle of the convergence test (multiple of the quantum) */
+  double tolerance;             /* cycle-to-cycle max |du|; 0 runs max_steps */
+  double rho0;                  /* initial equilibrium density */
+} hlb_request;
+
+typedef struct hlb_request_status {
+  int32_t state;                /* HLB_REQ_* */
+  int32_t cycles;
+  int64_t steps;
+  int64_t sites;
+  int64_t device_bytes;
+  double delta;                 /* last cycle-to-cycle max |du| (inf before two cycle ends) */
+  double t_submit, t_admit, t_done;
+} hlb_request_status;
+
+typedef struct hlb_server* hlb_server_handle;
+int hlb_server_create(int64_t budget_bytes, int64_t quantum_steps, hlb_server_handle* out);
+void hlb_server_destroy(hlb_server_handle s);
+/* Builds the request's lattice tables on the host and queues it. */
+int hlb_server_submit(hlb_server_handle s, const hlb_request* r, int64_t* id);
+int hlb_server_advance(hlb_server_handle s, int* running, int* queued);
+int hlb_server_status(hlb_server_handle s, int64_t id, hlb_request_status* out);
+/* Fields of a finished request, reference site order. */
+int hlb_server_fields(hlb_server_handle s, int64_t id, double* rho, double* u, double* wss);
+int hlb_server_release(hlb_server_handle s, int64_t id);
+int64_t hlb_server_site_steps(hlb_server_handle s);
+int64_t hlb_server_bytes_in_use(hlb_server_handle s);
+/* Device bytes a lattice holds once it has a state (admission accounting). */
+int64_t hlb_device_bytes(hlb_handle h);
+
 /* Iolet state (outlet = 0: inlet `index`, 1: outlet `index`):
  * out[0] density used by the last step's ghost sites (for Windkessel iolets,
  * the committed density), out[1] densityNew, out[2] flowRate, out[3]

@@ -144,6 +144,15 @@ class Case:
         outs = [H.iolet_desc(s.kind, **s.kw()) for s in self.outlets]
         return H.Lattice(self.geom, prm, ins, outs)
 
+    def submit(self, server: H.Server, **kw) -> int:
+        """Queues this case as an FP32-shifted request of `server`."""
+        prm = H.params(kernel=self.kernel, wall_bc=self.wall_bc, inlet_bc=self.inlet_bc,
+                       outlet_bc=self.outlet_bc, tau=self.tau, trt_lambda=self.trt_lambda,
+                       precision=H.FP32_SHIFTED)
+        ins = [H.iolet_desc(s.kind, **s.kw()) for s in self.inlets]
+        outs = [H.iolet_desc(s.kind, **s.kw()) for s in self.outlets]
+        return server.submit(self.geom, prm, ins, outs, **kw)
+
     def oracle(self):
         import oracle as O
         lat = O.OracleLattice(self.geom)
@@ -213,6 +222,46 @@ def ensemble_cases(count: int, seed: int = 2026, r_range=(6.0, 24.0), l_range=(6
 
     with ThreadPoolExecutor(workers) as ex:
         return list(ex.map(lambda d: stenosis_case(*d), draws))
+
+
+def serving_requests(count: int, seed: int = 7, min_sites: float = 2e4, max_sites: float = 5e6,
+                     period: float = 2000.0, workers: int = 1):
+    """Solve requests for the serving loop, drawn in order from one seeded
+    generator: kind (straight pipe, stenosed pipe, Y-bifurcation), a site
+    count log-uniform in [min_sites, max_sites], LBGK or TRT, tau U(0.6,0.9),
+    a cosine pressure inlet about 1 + amp with amplitude amp U(1e-4,1e-3) and
+    the given period, Nash outlets at rho 1.  Voxelised on `workers` threads."""
+    rng = np.random.default_rng(seed)
+    draws = []
+    for _ in range(count):
+        kind = int(rng.integers(0, 3))
+        n = float(np.exp(rng.uniform(np.log(min_sites), np.log(max_sites))))
+        kernel = H.KERNEL_LBGK if rng.uniform() < 0.5 else H.KERNEL_TRT
+        tau = float(rng.uniform(0.6, 0.9))
+        amp = float(rng.uniform(1e-4, 1e-3))
+        extra = float(rng.uniform(0.2, 0.7)) if kind == 1 else int(rng.integers(1 << 30)) if kind == 2 else 0.0
+        draws.append((kind, n, kernel, tau, amp, extra))
+
+    def build(d):
+        kind, n, kernel, tau, amp, extra = d
+        if kind < 2:
+            r = (n / (8.0 * math.pi)) ** (1.0 / 3.0)
+            return stenosis_case(r, int(8 * r), extra if kind == 1 else 0.0, tau, amp, period, kernel=kernel,
+                                 wall_bc=H.WALL_BFL)
+        sc = (n / 708000.0) ** (1.0 / 3.0)
+        g = H.Geometry.ybifurcation(24 * sc, 200 * sc, 19 * sc, 160 * sc, 30.0, 0.03, extra)
+        i0 = g.inlets()[0]
+        return Case(g, kernel=kernel, wall_bc=H.WALL_BFL, tau=tau,
+                    inlets=[IoletSpec(H.COSINE, i0["position"], i0["normal"], density_mean=1.0 + amp,
+                                      density_amp=amp, period=period)],
+                    outlets=[IoletSpec(H.COSINE, o["position"], o["normal"]) for o in g.outlets()])
+
+    if workers <= 1:
+        return [build(d) for d in draws]
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(build, draws))
 
 
 def sine_velocity_table(nsteps: int, peak: float, period: float) -> np.ndarray:

@@ -87,3 +87,24 @@ def test_batch_rejects_fp64_member_without_device():
         b.add(lat)
     with pytest.raises(H.HlbError, match="no such variant"):
         b.set_variant(6)
+
+
+def test_server_validates_requests_without_device():
+    from fixtures import cylinder_case
+
+    case = cylinder_case(radius=5.0, length=20, box_xy=16)
+    srv = H.Server(budget_bytes=1 << 30, quantum_steps=100)
+    i0, o0 = case.inlets[0], case.outlets[0]
+    ins = [H.iolet_desc(i0.kind, **i0.kw())]
+    outs = [H.iolet_desc(o0.kind, **o0.kw())]
+    with pytest.raises(H.HlbError, match="FP32_SHIFTED"):
+        srv.submit(case.geom, H.params(), ins, outs, max_steps=10)
+    with pytest.raises(H.HlbError, match="multiple of the quantum"):
+        srv.submit(case.geom, H.params(precision=H.FP32_SHIFTED), ins, outs, max_steps=10, period=150,
+                   tolerance=1e-6)
+    tiny = H.Server(budget_bytes=1000, quantum_steps=100)
+    with pytest.raises(H.HlbError, match="more than the budget"):
+        tiny.submit(case.geom, H.params(precision=H.FP32_SHIFTED), ins, outs, max_steps=10)
+    rid = srv.submit(case.geom, H.params(precision=H.FP32_SHIFTED), ins, outs, max_steps=10)
+    st = srv.status(rid)
+    assert st.state == H.REQ_QUEUED and st.sites == case.geom.n_sites and st.device_bytes > 0
