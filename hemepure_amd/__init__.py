@@ -142,6 +142,10 @@ _SIGS = {
     "hlb_timestep": (C.c_int64, [C.c_void_p]),
     "hlb_stability": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "hlb_iolet_state": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "hlb_monitor": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hlb_set_params": (C.c_int, [C.c_void_p, C.POINTER(Params)]),
+    "hlb_get_params": (C.c_int, [C.c_void_p, C.POINTER(Params)]),
+    "hlb_set_iolets": (C.c_int, [C.c_void_p, C.POINTER(IoletDesc), C.c_int, C.POINTER(IoletDesc), C.c_int]),
     "hlb_device_bytes": (C.c_int64, [C.c_void_p]),
     "hlb_server_create": (C.c_int, [C.c_int64, C.c_int64, C.POINTER(C.c_void_p)]),
     "hlb_server_destroy": (None, [C.c_void_p]),
@@ -176,6 +180,7 @@ _SIGS = {
     "hlb_geom_stenosed_pipe": (C.c_int, [C.c_double, C.c_int32, C.c_double, C.c_double,
                                          C.POINTER(C.c_void_p)]),
     "hlb_geom_view": (C.c_int, [C.c_void_p, C.POINTER(GeometryView)]),
+    "hlb_geom_read_gmy": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]),
     "hlb_geom_iolets": (C.c_int, [C.c_void_p, C.POINTER(IoletGeom), C.c_int]),
     "hlb_geom_free": (None, [C.c_void_p]),
 }
@@ -269,6 +274,14 @@ class Geometry:
             L.hlb_geom_free(gptr)
 
     @classmethod
+    def read_gmy(cls, path, gmy_plus: bool = False) -> "Geometry":
+        """A HemeLB .gmy (version 4) file; .gmy+ with gmy_plus.  The file holds
+        no iolet positions: give them from the case's XML."""
+        g = C.c_void_p()
+        _check(lib().hlb_geom_read_gmy(str(path).encode(), 1 if gmy_plus else 0, C.byref(g)))
+        return cls._from_native(g)
+
+    @classmethod
     def cylinder(cls, radius: float, length: int, box_xy: int) -> "Geometry":
         g = C.c_void_p()
         _check(lib().hlb_geom_cylinder(radius, length, box_xy, C.byref(g)))
@@ -341,16 +354,32 @@ def _desc_array(descs):
 class Lattice:
     """One handle: LatticeData + LBM + BoundaryValues of the reference, on one GPU."""
 
-    def __init__(self, geom: Geometry, prm: Params, inlets=(), outlets=()):
+    def __init__(self, geom: Geometry, prm: Params | None, inlets=(), outlets=()):
+        """prm None: tables only; configure with set_params / set_iolets."""
         self._geom = geom
         self._inlets = list(inlets)
         self._outlets = list(outlets)
         v = geom.view()
         h = C.c_void_p()
-        _check(lib().hlb_create(C.byref(v), C.byref(prm), _desc_array(self._inlets), len(self._inlets),
+        _check(lib().hlb_create(C.byref(v), C.byref(prm) if prm is not None else None,
+                                _desc_array(self._inlets), len(self._inlets),
                                 _desc_array(self._outlets), len(self._outlets), C.byref(h)))
         self._h = h
         self.n = int(lib().hlb_site_count(h))
+
+    def set_params(self, prm: Params):
+        _check(lib().hlb_set_params(self._h, C.byref(prm)))
+
+    def get_params(self) -> Params:
+        p = Params()
+        _check(lib().hlb_get_params(self._h, C.byref(p)))
+        return p
+
+    def set_iolets(self, inlets=(), outlets=()):
+        self._inlets = list(inlets)
+        self._outlets = list(outlets)
+        _check(lib().hlb_set_iolets(self._h, _desc_array(self._inlets), len(self._inlets),
+                                    _desc_array(self._outlets), len(self._outlets)))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -438,6 +467,12 @@ class Lattice:
         s = C.c_int()
         _check(lib().hlb_stability(self._h, C.byref(s)))
         return bool(s.value)
+
+    def monitor(self) -> dict:
+        """Device reductions of the last property step: min/max density, max |u|, mass."""
+        out = np.zeros(4, dtype=np.float64)
+        _check(lib().hlb_monitor(self._h, _ptr(out)))
+        return {"rho_min": out[0], "rho_max": out[1], "u_max": out[2], "mass": out[3]}
 
     def iolet_state(self, outlet: bool, index: int) -> np.ndarray:
         """(density, densityNew, flowRate, flowRateOld) of one iolet."""

@@ -19,7 +19,7 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libhemepure_amd.so"
 
-SOURCES = ["capi.hip", "batch.hip", "server.hip", "lbm_fp64.hip", "lbm_fp32.hip", "lattice.cpp", "geometry_gen.cpp"]
+SOURCES = ["capi.hip", "batch.hip", "server.hip", "lbm_fp64.hip", "lbm_fp32.hip", "lattice.cpp", "geometry_gen.cpp", "gmy_reader.cpp"]
 ARCH = os.environ.get("HLB_OFFLOAD_ARCH", "gfx950")
 
 
@@ -58,7 +58,7 @@ def build(verbose: bool = True) -> Path:
     with ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)]
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-lz", "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")

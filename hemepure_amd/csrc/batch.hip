@@ -89,12 +89,12 @@ int ensure_stream(hlb_batch* b) {
 int rebuild(hlb_batch* b) {
   HLB_CHECK(hipStreamSynchronize(b->stream));
   b->drop_graphs();
-  const int64_t B = fp32_variant_block(b->variant);
+  const int64_t per = fp32_variant_slabs(b->variant);
   std::vector<int32_t> map;
   std::vector<WkRef> refs;
   for (size_t i = 0; i < b->members.size(); ++i) {
     const hlb_lattice* m = b->members[i];
-    const int64_t tiles = (m->L.N + B - 1) / B;
+    const int64_t tiles = (m->nslab + per - 1) / per;
     map.insert(map.end(), (size_t)tiles, (int32_t)i);
     refs.insert(refs.end(), m->wk_refs_host.begin(), m->wk_refs_host.end());
   }
@@ -167,7 +167,7 @@ int check_members(hlb_batch* b, const char* who) {
 // Stages a chunk (<= kChunk steps): every member's iolet values and record.
 int stage_chunk(hlb_batch* b, int64_t n, int props_at) {
   HLB_CHECK(hipStreamSynchronize(b->stream));  // staging buffers are free again
-  const int64_t B = fp32_variant_block(b->variant);
+  const int64_t per = fp32_variant_slabs(b->variant);
   uint32_t wg = 0;
   for (size_t i = 0; i < b->members.size(); ++i) {
     hlb_lattice* m = b->members[i];
@@ -175,7 +175,7 @@ int stage_chunk(hlb_batch* b, int64_t n, int props_at) {
     if (rc != HLB_OK) return rc;
     Member32 r = member_record(m, props_at);
     r.wg_begin = wg;
-    wg += (uint32_t)((m->L.N + B - 1) / B);
+    wg += (uint32_t)((m->nslab + per - 1) / per);
     b->h_mem[i] = r;
   }
   HLB_CHECK(hipMemcpyAsync(b->mem.p, b->h_mem, sizeof(Member32) * b->members.size(), hipMemcpyHostToDevice,

@@ -76,8 +76,7 @@ StepArgs make_args(hlb_lattice* h, const double* ain, double* aout) {
     a.ain[j] = ain ? ain + (size_t)j * h->S : nullptr;
     a.aout[j] = aout ? aout + (size_t)j * h->S : nullptr;
   }
-  for (int j = 0; j < 18; ++j) a.pullp[j] = h->pull.p ? h->pull.p + (size_t)j * h->S : nullptr;
-  a.bidx = h->bidx.p;
+  a.bt = h->tables();
   a.bwall = h->bwall.p;
   a.biolet = h->biolet.p;
   a.bgi = h->bgi.p;
@@ -145,35 +144,79 @@ int side_iolet(const hlb_lattice* h, int64_t k) {
   return (c == 2 || c == 4) ? h->L.ioletId[k] : h->n_in + h->L.ioletId[k];
 }
 
-// The device order and the boundary / iolet slot lists.
+// The brick layout: the site grid cut into 8^3 tiles (independent of the
+// geometry's own block size), occupied tiles numbered in tile order (z
+// fastest) as bricks; slots, the neighbour rows, the active slabs and the
+// boundary / iolet lists in slot order.
 void device_order(hlb_lattice* h) {
   const HostLattice& L = h->L;
   const int64_t N = L.N;
-  h->perm.resize(N);
-  h->inv.resize(N);
-  const int64_t spb = (int64_t)L.bs * L.bs * L.bs;
-  int64_t d = 0;
-  for (size_t b = 0; b < L.blockSlot.size(); ++b) {
-    const int64_t slot = L.blockSlot[b];
-    if (slot < 0) continue;
-    for (int64_t s = 0; s < spb; ++s) {
-      const int32_t k = L.slotSites[slot * spb + s];
-      if (k < 0) continue;
-      h->perm[d] = k;
-      h->inv[k] = (int32_t)d;
-      ++d;
-    }
+  const int64_t ntx = ((int64_t)L.blocks[0] * L.bs + 7) / 8, nty = ((int64_t)L.blocks[1] * L.bs + 7) / 8,
+                ntz = ((int64_t)L.blocks[2] * L.bs + 7) / 8;
+  std::vector<int32_t> brickOf((size_t)(ntx * nty * ntz), -1);
+  for (int64_t k = 0; k < N; ++k) {
+    const int64_t t = ((int64_t)(L.coords[3 * k] >> 3) * nty + (L.coords[3 * k + 1] >> 3)) * ntz + (L.coords[3 * k + 2] >> 3);
+    brickOf[t] = 0;
   }
+  std::vector<int64_t> brickTile;
+  for (size_t t = 0; t < brickOf.size(); ++t)
+    if (brickOf[t] >= 0) {
+      brickOf[t] = (int32_t)brickTile.size();
+      brickTile.push_back((int64_t)t);
+    }
+  const int64_t NBr = (int64_t)brickTile.size();
+  h->nbrick = NBr;
+  h->S = (NBr + 1) * kBrickSites;
+  h->perm.assign((size_t)h->S, -1);
+  h->inv.assign((size_t)N, -1);
+  for (int64_t k = 0; k < N; ++k) {
+    const int32_t x = L.coords[3 * k], y = L.coords[3 * k + 1], z = L.coords[3 * k + 2];
+    const int64_t brick = brickOf[((int64_t)(x >> 3) * nty + (y >> 3)) * ntz + (z >> 3)];
+    const int64_t slot = brick * kBrickSites + ((x & 7) * 8 + (y & 7)) * 8 + (z & 7);
+    h->perm[slot] = (int32_t)k;
+    h->inv[k] = (int32_t)slot;
+  }
+  h->h_nbr.assign((size_t)(NBr + 1) * 27, (int32_t)NBr);
+  h->h_slab.clear();
+  h->h_fmask.clear();
+  h->h_bmask.clear();
+  h->h_bbase.clear();
   h->bsite.clear();
   h->isite.clear();
-  for (int64_t dd = 0; dd < N; ++dd) {
-    const int32_t k = h->perm[dd];
-    if (k >= L.off[1]) h->bsite.push_back(k);
-    if (k >= L.off[2]) h->isite.push_back(k);
+  for (int64_t k = 0; k < NBr; ++k) {
+    const int64_t t = brickTile[k];
+    const int64_t tx = t / (nty * ntz), ty = (t / ntz) % nty, tz = t % ntz;
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dz = -1; dz <= 1; ++dz) {
+          const int64_t x = tx + dx, y = ty + dy, z = tz + dz;
+          if (x < 0 || y < 0 || z < 0 || x >= ntx || y >= nty || z >= ntz) continue;
+          const int32_t nb = brickOf[(x * nty + y) * ntz + z];
+          if (nb >= 0) h->h_nbr[(size_t)k * 27 + (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)] = nb;
+        }
+    for (int x = 0; x < 8; ++x) {
+      uint64_t fm = 0, bm = 0;
+      const int32_t base = (int32_t)h->bsite.size();
+      for (int lane = 0; lane < kSlab; ++lane) {
+        const int32_t site = h->perm[k * kBrickSites + x * kSlab + lane];
+        if (site < 0) continue;
+        fm |= 1ull << lane;
+        if (site >= L.off[1]) {
+          bm |= 1ull << lane;
+          h->bsite.push_back(site);
+        }
+        if (site >= L.off[2]) h->isite.push_back(site);
+      }
+      if (fm == 0) continue;
+      h->h_slab.push_back((int32_t)(k * 8 + x));
+      h->h_fmask.push_back(fm);
+      h->h_bmask.push_back(bm);
+      h->h_bbase.push_back(base);
+    }
   }
+  h->nslab = (int64_t)h->h_slab.size();
   h->NB = (int64_t)h->bsite.size();
   h->NI = (int64_t)h->isite.size();
-  h->S = round_up(std::max<int64_t>(N, 1), 64);
   h->SB = round_up(std::max<int64_t>(h->NB, 1), 64);
   h->SI = round_up(std::max<int64_t>(h->NI, 1), 64);
 }
@@ -292,8 +335,6 @@ int upload_lattice(hlb_lattice* h) {
     HLB_CHECK(hipMemsetAsync(h->A[1].p, 0, sizeof(double) * Q * S, h->stream));
     HLB_CHECK(hipMemsetAsync(h->G.p, 0, sizeof(double) * Q * SI, h->stream));
   }
-  HLB_CHECK(h->pull.alloc((size_t)18 * S));
-  HLB_CHECK(h->bidx.alloc((size_t)S));
   HLB_CHECK(h->bwall.alloc((size_t)SB));
   HLB_CHECK(h->biolet.alloc((size_t)SB));
   HLB_CHECK(h->bgi.alloc((size_t)SB));
@@ -306,20 +347,29 @@ int upload_lattice(hlb_lattice* h) {
   HLB_CHECK(h->wss.alloc((size_t)S));
   HLB_CHECK(h->flag.alloc(1));
   HLB_CHECK(h->feq_vals.alloc(Q));
+  // Holes are never written: keep the field arrays defined there.
+  HLB_CHECK(hipMemsetAsync(h->rho.p, 0, sizeof(double) * S, h->stream));
+  HLB_CHECK(hipMemsetAsync(h->u.p, 0, sizeof(double) * 3 * S, h->stream));
+  HLB_CHECK(hipMemsetAsync(h->wss.p, 0, sizeof(double) * S, h->stream));
 
-  // Pull table in device slots, flag on plane 0, compact boundary index.
+  // Brick tables.
   {
-    std::vector<uint32_t> tmp((size_t)18 * S, 0);
-    for (int j = 0; j < 18; ++j)
-      for (int64_t d = 0; d < N; ++d) tmp[(size_t)j * S + d] = (uint32_t)h->inv[L.pull[(size_t)j * N + h->perm[d]]];
-    std::vector<int32_t> bidx(S, -1);
-    for (int64_t b = 0; b < NB; ++b) {
-      const int32_t d = h->inv[h->bsite[b]];
-      bidx[d] = (int32_t)b;
-      tmp[d] |= kBoundaryFlag;
+    const size_t ns = std::max<size_t>(1, h->h_slab.size());
+    HLB_CHECK(h->nbr.alloc(h->h_nbr.size()));
+    HLB_CHECK(h->slab.alloc(ns));
+    HLB_CHECK(h->fmask.alloc(ns));
+    HLB_CHECK(h->bmask.alloc(ns));
+    HLB_CHECK(h->bbase.alloc(ns));
+    HLB_CHECK(hipMemcpy(h->nbr.p, h->h_nbr.data(), sizeof(int32_t) * h->h_nbr.size(), hipMemcpyHostToDevice));
+    if (!h->h_slab.empty()) {
+      HLB_CHECK(hipMemcpy(h->slab.p, h->h_slab.data(), sizeof(int32_t) * h->h_slab.size(), hipMemcpyHostToDevice));
+      HLB_CHECK(hipMemcpy(h->fmask.p, h->h_fmask.data(), sizeof(uint64_t) * h->h_fmask.size(),
+                          hipMemcpyHostToDevice));
+      HLB_CHECK(hipMemcpy(h->bmask.p, h->h_bmask.data(), sizeof(uint64_t) * h->h_bmask.size(),
+                          hipMemcpyHostToDevice));
+      HLB_CHECK(hipMemcpy(h->bbase.p, h->h_bbase.data(), sizeof(int32_t) * h->h_bbase.size(),
+                          hipMemcpyHostToDevice));
     }
-    HLB_CHECK(hipMemcpy(h->pull.p, tmp.data(), sizeof(uint32_t) * tmp.size(), hipMemcpyHostToDevice));
-    HLB_CHECK(hipMemcpy(h->bidx.p, bidx.data(), sizeof(int32_t) * bidx.size(), hipMemcpyHostToDevice));
   }
   // Boundary arrays in device order.
   {
@@ -404,6 +454,7 @@ void advance_state(hlb_lattice* h, int64_t n) {
 // inspected on a machine without a GPU.
 int ensure_device(hlb_lattice* h) {
   if (h->device_ready) return HLB_OK;
+  if (!h->iolets_set) return fail(HLB_EINVAL, "the lattice has no iolets configured (hlb_set_iolets)");
   HLB_CHECK(hipGetDevice(&h->device));
   HLB_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HLB_CHECK(hipEventCreate(&h->ev0));
@@ -416,34 +467,141 @@ int ensure_device(hlb_lattice* h) {
 
 namespace {
 
-int validate_params(const hlb_params& p, const hlb_iolet_desc* in, int nin, const hlb_iolet_desc* out, int nout) {
-  if (p.kernel < HLB_KERNEL_LBGK || p.kernel > HLB_KERNEL_LES) return fail(HLB_EINVAL, "hlb_create: unknown kernel");
-  if (p.wall_bc != HLB_WALL_SBB && p.wall_bc != HLB_WALL_BFL) return fail(HLB_EINVAL, "hlb_create: unknown wall BC");
+int validate_core(const hlb_params& p, const char* who) {
+  const std::string w(who);
+  if (p.kernel < HLB_KERNEL_LBGK || p.kernel > HLB_KERNEL_LES) return fail(HLB_EINVAL, w + ": unknown kernel");
+  if (p.wall_bc != HLB_WALL_SBB && p.wall_bc != HLB_WALL_BFL) return fail(HLB_EINVAL, w + ": unknown wall BC");
   for (int bc : {p.inlet_bc, p.outlet_bc})
-    if (bc != HLB_IOLET_NASH && bc != HLB_IOLET_LADD) return fail(HLB_EINVAL, "hlb_create: unknown iolet BC");
+    if (bc != HLB_IOLET_NASH && bc != HLB_IOLET_LADD) return fail(HLB_EINVAL, w + ": unknown iolet BC");
   if (p.precision != HLB_FP64 && p.precision != HLB_FP32_SHIFTED)
-    return fail(HLB_EINVAL, "hlb_create: unknown storage precision");
-  if (!(p.tau > 0.5)) return fail(HLB_EINVAL, "hlb_create: tau must exceed 1/2");
+    return fail(HLB_EINVAL, w + ": unknown storage precision");
+  if (!(p.tau > 0.5)) return fail(HLB_EINVAL, w + ": tau must exceed 1/2");
+  return HLB_OK;
+}
+
+int validate_iolets(const hlb_params& p, const hlb_iolet_desc* in, int nin, const hlb_iolet_desc* out, int nout,
+                    const char* who) {
+  const std::string w(who);
   for (int side = 0; side < 2; ++side) {
     const hlb_iolet_desc* v = side ? out : in;
     const int n = side ? nout : nin;
     const int bc = side ? p.outlet_bc : p.inlet_bc;
-    if (n > 0 && !v) return fail(HLB_EINVAL, "hlb_create: null iolet array");
+    if (n > 0 && !v) return fail(HLB_EINVAL, w + ": null iolet array");
     for (int i = 0; i < n; ++i) {
       const int k = v[i].kind;
-      if (k < HLB_IOLET_COSINE || k > HLB_IOLET_WK3) return fail(HLB_EINVAL, "hlb_create: unknown iolet kind");
+      if (k < HLB_IOLET_COSINE || k > HLB_IOLET_WK3) return fail(HLB_EINVAL, w + ": unknown iolet kind");
       if ((k == HLB_IOLET_WK2 || k == HLB_IOLET_WK3) && bc != HLB_IOLET_NASH)
-        return fail(HLB_EINVAL, "hlb_create: a Windkessel iolet needs the Nash pressure boundary");
+        return fail(HLB_EINVAL, w + ": a Windkessel iolet needs the Nash pressure boundary");
       const bool vel = (k == HLB_IOLET_PARABOLIC || k == HLB_IOLET_FILEVEL);
       if (bc == HLB_IOLET_LADD && !vel)
-        return fail(HLB_EINVAL, "hlb_create: the Ladd boundary needs a velocity iolet (LaddIoletDelegate.h:51-53)");
-      if (vel && !(v[i].radius > 0.0)) return fail(HLB_EINVAL, "hlb_create: velocity iolet without radius");
+        return fail(HLB_EINVAL, w + ": the Ladd boundary needs a velocity iolet (LaddIoletDelegate.h:51-53)");
+      if (vel && !(v[i].radius > 0.0)) return fail(HLB_EINVAL, w + ": velocity iolet without radius");
       if (k == HLB_IOLET_FILEVEL && (!v[i].table || v[i].table_len <= 0))
-        return fail(HLB_EINVAL, "hlb_create: file velocity iolet without a table");
-      if (k == HLB_IOLET_COSINE && !(v[i].period != 0.0))
-        return fail(HLB_EINVAL, "hlb_create: cosine iolet with zero period");
+        return fail(HLB_EINVAL, w + ": file velocity iolet without a table");
+      if (k == HLB_IOLET_COSINE && !(v[i].period != 0.0)) return fail(HLB_EINVAL, w + ": cosine iolet with zero period");
     }
   }
+  return HLB_OK;
+}
+
+// LbmParameters (LbmParameters.h:29-45) and the TRT / LES constants.  Once
+// the lattice has a device state only tau and the TRT magic parameter may
+// change (LbmParameters::Update).
+int configure_params(hlb_lattice* h, const hlb_params& p, const char* who) {
+  int rc = validate_core(p, who);
+  if (rc != HLB_OK) return rc;
+  const hlb_params& o = h->params;
+  if (h->device_ready && (p.kernel != o.kernel || p.wall_bc != o.wall_bc || p.inlet_bc != o.inlet_bc ||
+                          p.outlet_bc != o.outlet_bc || p.precision != o.precision))
+    return fail(HLB_EINVAL, std::string(who) + ": only tau and trt_lambda can change once the lattice has a state");
+  if (h->iolets_set && !h->device_ready) {
+    std::vector<hlb_iolet_desc> in, out;
+    for (int g = 0; g < (int)h->iolets.size(); ++g) (g < h->n_in ? in : out).push_back(h->iolets[g].d);
+    for (auto* v : {&in, &out})
+      for (size_t i = 0; i < v->size(); ++i) {
+        const HostIolet& io = h->iolets[(v == &in ? 0 : h->n_in) + i];
+        if (!io.table.empty()) {
+          (*v)[i].table = io.table.data();
+          (*v)[i].table_len = (int64_t)io.table.size();
+        }
+      }
+    rc = validate_iolets(p, in.data(), (int)in.size(), out.data(), (int)out.size(), who);
+    if (rc != HLB_OK) return rc;
+  }
+  h->params = p;
+  const double tau = p.tau;
+  h->omega = -1.0 / tau;
+  h->half_omega = 0.5 * h->omega;
+  const double tau_minus = 0.5 + p.trt_lambda / (tau - 0.5);
+  h->half_omega_minus = 0.5 * (-1.0 / tau_minus);
+  {
+    const double rho1 = 1.0, C_smag = 0.1;
+    double k = 18.0 * 1.4142135623730950488016887242097;
+    k = k * rho1;
+    k = k * C_smag;
+    k = k * C_smag;
+    h->les_k = k;
+  }
+  if (h->iolets_set && !h->device_ready) return ladd_factors(h);
+  return HLB_OK;
+}
+
+// BoundaryValues::BoundaryValues (src/lb/iolets/BoundaryValues.cc:20-81) and
+// LBM::PrepareBoundaryObjects (src/lb/lb.hpp:119-143).
+int configure_iolets(hlb_lattice* h, const hlb_iolet_desc* inlets, int n_inlets, const hlb_iolet_desc* outlets,
+                     int n_outlets, const char* who) {
+  const std::string w(who);
+  if (h->device_ready) return fail(HLB_EINVAL, w + ": the iolets are fixed once the lattice has a state");
+  if (n_inlets < 0 || n_outlets < 0) return fail(HLB_EINVAL, w + ": negative iolet count");
+  if (n_inlets + n_outlets > kMaxIolets)
+    return fail(HLB_EINVAL, w + ": at most " + std::to_string(kMaxIolets) + " iolets per lattice");
+  int rc = validate_iolets(h->params, inlets, n_inlets, outlets, n_outlets, who);
+  if (rc != HLB_OK) return rc;
+  // Iolet ids on sites must name a configured iolet of the site's side.
+  for (int64_t k = h->L.off[2]; k < h->L.N; ++k) {
+    const int c = h->L.collision_class(k);
+    const int lim = (c == 2 || c == 4) ? n_inlets : n_outlets;
+    if (h->L.ioletId[k] < 0 || h->L.ioletId[k] >= lim)
+      return fail(HLB_EINVAL, w + ": a site names iolet " + std::to_string(h->L.ioletId[k]) + " but only " +
+                                  std::to_string(lim) + " are configured on its side");
+  }
+  h->iolets.clear();
+  h->n_in = n_inlets;
+  h->n_out = n_outlets;
+  for (int side = 0; side < 2; ++side) {
+    const hlb_iolet_desc* v = side ? outlets : inlets;
+    const int n = side ? n_outlets : n_inlets;
+    for (int i = 0; i < n; ++i) {
+      HostIolet io;
+      io.d = v[i];
+      double nx = v[i].normal[0], ny = v[i].normal[1], nz = v[i].normal[2];
+      if (!(nx == 0.0 && ny == 0.0 && nz == 0.0)) {
+        const double mag = std::sqrt(nx * nx + ny * ny + nz * nz);
+        nx /= mag;
+        ny /= mag;
+        nz /= mag;
+      }
+      io.n[0] = nx;
+      io.n[1] = ny;
+      io.n[2] = nz;
+      io.nf[0] = (float)nx;
+      io.nf[1] = (float)ny;
+      io.nf[2] = (float)nz;
+      if (v[i].kind == HLB_IOLET_FILEVEL) io.table.assign(v[i].table, v[i].table + v[i].table_len);
+      io.d.table = nullptr;
+      h->iolets.push_back(std::move(io));
+    }
+  }
+  // Every iolet learns the minimum density any iolet can impose.
+  double minDensity = DBL_MAX;
+  for (const HostIolet& io : h->iolets)
+    minDensity = std::min(minDensity, io.d.kind == HLB_IOLET_COSINE ? io.d.density_mean - io.d.density_amp : 1.0);
+  for (HostIolet& io : h->iolets) io.minSimDensity = minDensity;
+  rc = ladd_factors(h);
+  if (rc != HLB_OK) return rc;
+  h->any_wk = false;
+  windkessel_setup(h);
+  h->iolets_set = true;
   return HLB_OK;
 }
 
@@ -455,8 +613,7 @@ Member32 member_record(const hlb_lattice* h, int props_at) {
   m.A[0] = h->A32[0].p;
   m.A[1] = h->A32[1].p;
   m.F = h->F32.p;
-  m.pull = h->pull.p;
-  m.bidx = h->bidx.p;
+  m.bt = h->tables();
   m.bwall = h->bwall.p;
   m.biolet = h->biolet.p;
   m.bgi = h->bgi.p;
@@ -498,9 +655,9 @@ Member32 member_record(const hlb_lattice* h, int props_at) {
 
 namespace {
 
-uint32_t member_grid(int64_t N) {
-  const int64_t B = fp32_variant_block(kFp32DefaultVariant);
-  const uint32_t tiles = (uint32_t)((N + B - 1) / B);
+uint32_t member_grid(int64_t nslab) {
+  const int64_t per = fp32_variant_slabs(kFp32DefaultVariant);
+  const uint32_t tiles = (uint32_t)((nslab + per - 1) / per);
   return (tiles + 7) / 8 * 8;
 }
 
@@ -537,7 +694,7 @@ int enqueue_steps(hlb_lattice* h, int64_t n, bool props_last) {
   if (h->fp32()) {
     int rc = upload_member(h, props_last ? (int)(n - 1) : -1);
     if (rc != HLB_OK) return rc;
-    const uint32_t grid = member_grid(h->L.N);
+    const uint32_t grid = member_grid(h->nslab);
     for (int64_t s = 0; s < n; ++s) {
       HLB_CHECK(launch_wk_begin(h->wk_refs.p, nwk, (int)s, h->stream));
       HLB_CHECK(launch_step_fp32(h->mem.p, nullptr, grid, (int)s, kFp32DefaultVariant, h->stream));
@@ -590,7 +747,7 @@ int materialise_f(hlb_lattice* h) {
     HLB_CHECK(hipStreamSynchronize(h->stream));
     int rc = upload_member(h, -1);
     if (rc != HLB_OK) return rc;
-    HLB_CHECK(launch_gather_fp32(h->mem.p, h->L.N, h->F32.p, h->stream));
+    HLB_CHECK(launch_gather_fp32(h->mem.p, h->nslab, h->F32.p, h->stream));
     return HLB_OK;
   }
   if (!h->F.p) HLB_CHECK(h->F.alloc((size_t)Q * h->S));
@@ -609,79 +766,61 @@ extern "C" const char* hlb_last_error(void) { return last_error().c_str(); }
 extern "C" int hlb_create(const hlb_geometry* geom, const hlb_params* params, const hlb_iolet_desc* inlets,
                           int n_inlets, const hlb_iolet_desc* outlets, int n_outlets, hlb_handle* out) {
   try {
-    if (!geom || !params || !out) return fail(HLB_EINVAL, "hlb_create: null argument");
-    if (n_inlets < 0 || n_outlets < 0) return fail(HLB_EINVAL, "hlb_create: negative iolet count");
-    int rc = validate_params(*params, inlets, n_inlets, outlets, n_outlets);
+    if (!geom || !out) return fail(HLB_EINVAL, "hlb_create: null argument");
+    hlb_params p{};
+    p.kernel = HLB_KERNEL_LBGK;
+    p.wall_bc = HLB_WALL_SBB;
+    p.inlet_bc = p.outlet_bc = HLB_IOLET_NASH;
+    p.precision = HLB_FP64;
+    p.tau = 1.0;
+    p.trt_lambda = 3.0 / 16.0;
+    if (params) p = *params;
+    int rc = validate_core(p, "hlb_create");
     if (rc != HLB_OK) return rc;
+    if (params) {
+      if (n_inlets < 0 || n_outlets < 0) return fail(HLB_EINVAL, "hlb_create: negative iolet count");
+      rc = validate_iolets(p, inlets, n_inlets, outlets, n_outlets, "hlb_create");
+      if (rc != HLB_OK) return rc;
+    }
     auto h = std::make_unique<hlb_lattice>();
-    h->params = *params;
-    h->n_in = n_inlets;
-    h->n_out = n_outlets;
     rc = build_lattice(*geom, h->L);
     if (rc != HLB_OK) return rc;
-    // Iolet ids on sites must name a configured iolet of the site's side.
-    for (int64_t k = h->L.off[2]; k < h->L.N; ++k) {
-      const int c = h->L.collision_class(k);
-      const int lim = (c == 2 || c == 4) ? n_inlets : n_outlets;
-      if (h->L.ioletId[k] < 0 || h->L.ioletId[k] >= lim)
-        return fail(HLB_EINVAL, "hlb_create: a site names iolet " + std::to_string(h->L.ioletId[k]) +
-                                    " but only " + std::to_string(lim) + " are configured on its side");
-    }
-    for (int side = 0; side < 2; ++side) {
-      const hlb_iolet_desc* v = side ? outlets : inlets;
-      const int n = side ? n_outlets : n_inlets;
-      for (int i = 0; i < n; ++i) {
-        HostIolet io;
-        io.d = v[i];
-        double nx = v[i].normal[0], ny = v[i].normal[1], nz = v[i].normal[2];
-        if (!(nx == 0.0 && ny == 0.0 && nz == 0.0)) {
-          const double mag = std::sqrt(nx * nx + ny * ny + nz * nz);
-          nx /= mag;
-          ny /= mag;
-          nz /= mag;
-        }
-        io.n[0] = nx;
-        io.n[1] = ny;
-        io.n[2] = nz;
-        io.nf[0] = (float)nx;
-        io.nf[1] = (float)ny;
-        io.nf[2] = (float)nz;
-        if (v[i].kind == HLB_IOLET_FILEVEL) io.table.assign(v[i].table, v[i].table + v[i].table_len);
-        io.d.table = nullptr;
-        h->iolets.push_back(std::move(io));
-      }
-    }
-    // LBM::PrepareBoundaryObjects (src/lb/lb.hpp:119-143): every iolet learns
-    // the minimum density any iolet can impose.
-    double minDensity = DBL_MAX;
-    for (const HostIolet& io : h->iolets)
-      minDensity = std::min(minDensity, io.d.kind == HLB_IOLET_COSINE ? io.d.density_mean - io.d.density_amp : 1.0);
-    for (HostIolet& io : h->iolets) io.minSimDensity = minDensity;
-    // LbmParameters (LbmParameters.h:29-45) and the TRT / LES constants.
-    const double tau = params->tau;
-    h->omega = -1.0 / tau;
-    h->half_omega = 0.5 * h->omega;
-    const double tau_minus = 0.5 + params->trt_lambda / (tau - 0.5);
-    h->half_omega_minus = 0.5 * (-1.0 / tau_minus);
-    {
-      const double rho1 = 1.0, C_smag = 0.1;
-      double k = 18.0 * 1.4142135623730950488016887242097;
-      k = k * rho1;
-      k = k * C_smag;
-      k = k * C_smag;
-      h->les_k = k;
-    }
-    if ((int)h->iolets.size() > kMaxIolets)
-      return fail(HLB_EINVAL, "hlb_create: at most " + std::to_string(kMaxIolets) + " iolets per lattice");
     device_order(h.get());
-    rc = ladd_factors(h.get());
+    if (h->S > (int64_t(1) << 29))
+      return fail(HLB_EINVAL, "hlb_create: the geometry needs " + std::to_string(h->S) +
+                                  " device slots (occupied 8^3 tiles x 512), more than 2^29");
+    rc = configure_params(h.get(), p, "hlb_create");
     if (rc != HLB_OK) return rc;
-    windkessel_setup(h.get());
+    if (params) {
+      rc = configure_iolets(h.get(), inlets, n_inlets, outlets, n_outlets, "hlb_create");
+      if (rc != HLB_OK) return rc;
+    }
     *out = h.release();
     return HLB_OK;
   } catch (const std::bad_alloc&) {
     return fail(HLB_EINVAL, "hlb_create: out of host memory");
   }
+}
+
+extern "C" int hlb_set_params(hlb_handle h, const hlb_params* params) {
+  if (!h || !params) return fail(HLB_EINVAL, "hlb_set_params: null argument");
+  return configure_params(h, *params, "hlb_set_params");
+}
+
+extern "C" int hlb_set_iolets(hlb_handle h, const hlb_iolet_desc* inlets, int n_inlets, const hlb_iolet_desc* outlets,
+                              int n_outlets) {
+  if (!h) return fail(HLB_EINVAL, "hlb_set_iolets: null handle");
+  try {
+    return configure_iolets(h, inlets, n_inlets, outlets, n_outlets, "hlb_set_iolets");
+  } catch (const std::bad_alloc&) {
+    return fail(HLB_EINVAL, "hlb_set_iolets: out of host memory");
+  }
+}
+
+extern "C" int hlb_get_params(hlb_handle h, hlb_params* out) {
+  if (!h || !out) return fail(HLB_EINVAL, "hlb_get_params: null argument");
+  *out = h->params;
+  return HLB_OK;
 }
 
 extern "C" void hlb_destroy(hlb_handle h) {
@@ -704,7 +843,7 @@ extern "C" int hlb_init_equilibrium(hlb_handle h, double rho0, int64_t t0) {
     for (int j = 0; j < Q; ++j) g[j] = (float)(feq[j] - Wt(j));
     if (!h->F32.p) HLB_CHECK(h->F32.alloc((size_t)Q * h->S));
     HLB_CHECK(hipMemcpyAsync(h->feq32.p, g, sizeof g, hipMemcpyHostToDevice, h->stream));
-    HLB_CHECK(launch_fill_fp32(h->F32.p, h->L.N, h->S, h->feq32.p, h->stream));
+    HLB_CHECK(launch_fill_fp32(h->F32.p, h->S, h->S, h->feq32.p, h->stream));
     HLB_CHECK(hipStreamSynchronize(h->stream));
     h->have_post = false;
     h->f_valid = true;
@@ -713,7 +852,7 @@ extern "C" int hlb_init_equilibrium(hlb_handle h, double rho0, int64_t t0) {
   }
   if (!h->F.p) HLB_CHECK(h->F.alloc((size_t)Q * h->S));
   HLB_CHECK(hipMemcpyAsync(h->feq_vals.p, feq, sizeof feq, hipMemcpyHostToDevice, h->stream));
-  HLB_CHECK(launch_fill_fp64(h->F.p, h->L.N, h->S, h->feq_vals.p, h->stream));
+  HLB_CHECK(launch_fill_fp64(h->F.p, h->S, h->S, h->feq_vals.p, h->stream));
   HLB_CHECK(hipStreamSynchronize(h->stream));
   h->have_post = false;
   h->f_valid = true;
@@ -731,8 +870,9 @@ extern "C" int hlb_upload_f(hlb_handle h, const double* f, int64_t t0) {
   const int64_t N = h->L.N, S = h->S;
   if (h->fp32()) {
     std::vector<float> soa((size_t)Q * S, 0.0f);
-    for (int64_t d = 0; d < N; ++d) {
-      const double* src = f + (size_t)h->perm[d] * Q;
+    for (int64_t k = 0; k < N; ++k) {
+      const double* src = f + (size_t)k * Q;
+      const int64_t d = h->inv[k];
       for (int j = 0; j < Q; ++j) soa[(size_t)j * S + d] = (float)(src[j] - Wt(j));
     }
     if (!h->F32.p) HLB_CHECK(h->F32.alloc((size_t)Q * S));
@@ -743,8 +883,9 @@ extern "C" int hlb_upload_f(hlb_handle h, const double* f, int64_t t0) {
     return HLB_OK;
   }
   std::vector<double> soa((size_t)Q * S, 0.0);
-  for (int64_t d = 0; d < N; ++d) {
-    const double* src = f + (size_t)h->perm[d] * Q;
+  for (int64_t k = 0; k < N; ++k) {
+    const double* src = f + (size_t)k * Q;
+    const int64_t d = h->inv[k];
     for (int j = 0; j < Q; ++j) soa[(size_t)j * S + d] = src[j];
   }
   if (!h->F.p) HLB_CHECK(h->F.alloc((size_t)Q * S));
@@ -795,8 +936,9 @@ extern "C" int hlb_download_f(hlb_handle h, double* f) {
     std::vector<float> soa((size_t)Q * S);
     HLB_CHECK(hipMemcpyAsync(soa.data(), h->F32.p, sizeof(float) * soa.size(), hipMemcpyDeviceToHost, h->stream));
     HLB_CHECK(hipStreamSynchronize(h->stream));
-    for (int64_t d = 0; d < N; ++d) {
-      double* dst = f + (size_t)h->perm[d] * Q;
+    for (int64_t k = 0; k < N; ++k) {
+      double* dst = f + (size_t)k * Q;
+      const int64_t d = h->inv[k];
       for (int j = 0; j < Q; ++j) dst[j] = (double)soa[(size_t)j * S + d] + Wt(j);
     }
     return HLB_OK;
@@ -804,8 +946,9 @@ extern "C" int hlb_download_f(hlb_handle h, double* f) {
   std::vector<double> soa((size_t)Q * S);
   HLB_CHECK(hipMemcpyAsync(soa.data(), h->F.p, sizeof(double) * soa.size(), hipMemcpyDeviceToHost, h->stream));
   HLB_CHECK(hipStreamSynchronize(h->stream));
-  for (int64_t d = 0; d < N; ++d) {
-    double* dst = f + (size_t)h->perm[d] * Q;
+  for (int64_t k = 0; k < N; ++k) {
+    double* dst = f + (size_t)k * Q;
+    const int64_t d = h->inv[k];
     for (int j = 0; j < Q; ++j) dst[j] = soa[(size_t)j * S + d];
   }
   return HLB_OK;
@@ -818,17 +961,17 @@ extern "C" int hlb_get_fields(hlb_handle h, double* rho, double* u, double* wss)
   HLB_CHECK(hipStreamSynchronize(h->stream));
   std::vector<double> tmp((size_t)3 * S);
   if (rho) {
-    HLB_CHECK(hipMemcpy(tmp.data(), h->rho.p, sizeof(double) * N, hipMemcpyDeviceToHost));
-    for (int64_t d = 0; d < N; ++d) rho[h->perm[d]] = tmp[d];
+    HLB_CHECK(hipMemcpy(tmp.data(), h->rho.p, sizeof(double) * S, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < N; ++k) rho[k] = tmp[h->inv[k]];
   }
   if (wss) {
-    HLB_CHECK(hipMemcpy(tmp.data(), h->wss.p, sizeof(double) * N, hipMemcpyDeviceToHost));
-    for (int64_t d = 0; d < N; ++d) wss[h->perm[d]] = tmp[d];
+    HLB_CHECK(hipMemcpy(tmp.data(), h->wss.p, sizeof(double) * S, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < N; ++k) wss[k] = tmp[h->inv[k]];
   }
   if (u) {
     HLB_CHECK(hipMemcpy(tmp.data(), h->u.p, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
-    for (int64_t d = 0; d < N; ++d)
-      for (int a = 0; a < 3; ++a) u[3 * (size_t)h->perm[d] + a] = tmp[(size_t)a * S + d];
+    for (int64_t k = 0; k < N; ++k)
+      for (int a = 0; a < 3; ++a) u[3 * (size_t)k + a] = tmp[(size_t)a * S + h->inv[k]];
   }
   return HLB_OK;
 }
@@ -841,12 +984,25 @@ extern "C" int hlb_stability(hlb_handle h, int* stable) {
   int rc = materialise_f(h);
   if (rc != HLB_OK) return rc;
   HLB_CHECK(hipMemsetAsync(h->flag.p, 0, sizeof(int), h->stream));
-  if (h->fp32()) HLB_CHECK(launch_unstable_fp32(h->F32.p, h->L.N, h->S, h->flag.p, h->stream));
-  else HLB_CHECK(launch_unstable_fp64(h->F.p, h->L.N, h->S, h->flag.p, h->stream));
+  if (h->fp32()) HLB_CHECK(launch_unstable_fp32(h->tables(), h->F32.p, h->S, h->flag.p, h->stream));
+  else HLB_CHECK(launch_unstable_fp64(h->tables(), h->F.p, h->S, h->flag.p, h->stream));
   int flag = 0;
   HLB_CHECK(hipMemcpyAsync(&flag, h->flag.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HLB_CHECK(hipStreamSynchronize(h->stream));
   *stable = flag ? 0 : 1;
+  return HLB_OK;
+}
+
+extern "C" int hlb_monitor(hlb_handle h, double out[4]) {
+  if (!h || !out) return fail(HLB_EINVAL, "hlb_monitor: null argument");
+  if (!h->device_ready) return fail(HLB_EINVAL, "hlb_monitor: no state");
+  if (!h->mon_scratch.p) {
+    HLB_CHECK(h->mon_scratch.alloc(monitor_scratch(h->nslab)));
+    HLB_CHECK(h->mon_out.alloc(4));
+  }
+  HLB_CHECK(launch_monitor(h->tables(), h->rho.p, h->u.p, h->S, h->mon_scratch.p, h->mon_out.p, h->stream));
+  HLB_CHECK(hipMemcpyAsync(out, h->mon_out.p, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HLB_CHECK(hipStreamSynchronize(h->stream));
   return HLB_OK;
 }
 
@@ -870,13 +1026,21 @@ extern "C" int hlb_neighbour_tables(hlb_handle h, int64_t* push, uint32_t* pull)
   if (pull && !h->device_ready) {
     std::memcpy(pull, h->L.pull.data(), sizeof(uint32_t) * h->L.pull.size());
   } else if (pull) {
-    // Read back the device table and express it in the reference's order.
+    // The slots the device kernels gather from, computed by the same code
+    // on the device, expressed in the reference's order (a hole reads as
+    // 0xffffffff).
     const int64_t N = h->L.N, S = h->S;
+    DevBuf<uint32_t> dev;
+    HLB_CHECK(dev.alloc((size_t)18 * S));
+    HLB_CHECK(launch_upstream_table(h->tables(), h->bwall.p, h->biolet.p, dev.p, S, h->stream));
     std::vector<uint32_t> tmp((size_t)18 * S);
-    HLB_CHECK(hipMemcpy(tmp.data(), h->pull.p, sizeof(uint32_t) * tmp.size(), hipMemcpyDeviceToHost));
+    HLB_CHECK(hipMemcpyAsync(tmp.data(), dev.p, sizeof(uint32_t) * tmp.size(), hipMemcpyDeviceToHost, h->stream));
+    HLB_CHECK(hipStreamSynchronize(h->stream));
     for (int j = 0; j < 18; ++j)
-      for (int64_t d = 0; d < N; ++d)
-        pull[(size_t)j * N + h->perm[d]] = (uint32_t)h->perm[tmp[(size_t)j * S + d] & kIndexMask];
+      for (int64_t k = 0; k < N; ++k) {
+        const uint32_t v = tmp[(size_t)j * S + h->inv[k]];
+        pull[(size_t)j * N + k] = v < (uint32_t)S ? (uint32_t)h->perm[v] : 0xffffffffu;
+      }
   }
   return HLB_OK;
 }

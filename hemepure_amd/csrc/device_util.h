@@ -5,6 +5,9 @@
 
 #include <cstdint>
 
+#include "d3q19.h"
+#include "lbm_kernels.h"
+
 namespace hlb {
 
 constexpr int kBlock = 256;  // threads (device slots) per workgroup tile
@@ -47,5 +50,73 @@ __device__ __forceinline__ uint32_t xcd_tile() {
 }
 
 __device__ __forceinline__ uint32_t site_index() { return xcd_tile() * kBlock + threadIdx.x; }
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// One lane of a slab: its slot, block-local coordinates and boundary index.
+struct Lane {
+  uint32_t slot;
+  int x, y, z;       // x is wave-uniform
+  bool boundary;
+  uint32_t b;        // compact boundary index (valid when boundary)
+  const int32_t* row;  // the brick's 27 neighbour bricks (wave-uniform)
+};
+
+// Decodes slab record r for this lane; false for a hole or a record past the
+// end (the lane has nothing to do).  r must be wave-uniform.
+__device__ __forceinline__ bool slab_lane(const BrickTables& bt, uint32_t r, Lane& L) {
+  if (r >= (uint64_t)bt.nslab) return false;
+  const uint32_t lane = threadIdx.x & 63u;
+  const int32_t sid = __builtin_amdgcn_readfirstlane(ldp(bt.slab, r));
+  const uint64_t fm = uniform64(ldp(bt.fmask, r));
+  if (!((fm >> lane) & 1u)) return false;
+  const uint64_t bm = uniform64(ldp(bt.bmask, r));
+  const uint32_t brick = (uint32_t)sid >> 3;
+  L.x = sid & 7;
+  L.y = (int)(lane >> 3);
+  L.z = (int)(lane & 7u);
+  L.slot = brick * (uint32_t)kBrickSites + (uint32_t)L.x * 64u + lane;
+  L.boundary = (bm >> lane) & 1u;
+  L.b = 0;
+  if (L.boundary) {
+    const uint64_t below = bm & ((1ull << lane) - 1ull);
+    L.b = (uint32_t)__builtin_amdgcn_readfirstlane(ldp(bt.bbase, r)) + (uint32_t)__builtin_popcountll(below);
+  }
+  L.row = bt.nbr + (int64_t)brick * 27;
+  return true;
+}
+
+// Slot of the upstream site x - c_J of a lane (pull direction J).  The brick
+// holding it is one of at most four candidates read from the uniform
+// neighbour row; the lane picks by whether its y / z step leaves the block.
+template <int J>
+__device__ __forceinline__ uint32_t upstream_slot(const Lane& L) {
+  constexpr int ox = -CX[J], oy = -CY[J], oz = -CZ[J];
+  const bool xo = ox != 0 && (unsigned)(L.x + ox) > 7u;  // wave-uniform
+  const bool yo = oy != 0 && (unsigned)(L.y + oy) > 7u;
+  const bool zo = oz != 0 && (unsigned)(L.z + oz) > 7u;
+  constexpr int ix = (ox + 1) * 9, iy = (oy + 1) * 3, iz = oz + 1;
+  // Row entries for (dx, dy, dz) with each component either 0 or the step.
+  const int32_t c00 = xo ? ldp(L.row, ix + 4) : ldp(L.row, 13);
+  int32_t brick = c00;
+  if constexpr (oy != 0 && oz != 0) {
+    const int32_t c10 = xo ? ldp(L.row, ix + iy + 1) : ldp(L.row, 9 + iy + 1);
+    const int32_t c01 = xo ? ldp(L.row, ix + 3 + iz) : ldp(L.row, 9 + 3 + iz);
+    const int32_t c11 = xo ? ldp(L.row, ix + iy + iz) : ldp(L.row, 9 + iy + iz);
+    brick = yo ? (zo ? c11 : c10) : (zo ? c01 : c00);
+  } else if constexpr (oy != 0) {
+    const int32_t c10 = xo ? ldp(L.row, ix + iy + 1) : ldp(L.row, 9 + iy + 1);
+    brick = yo ? c10 : c00;
+  } else if constexpr (oz != 0) {
+    const int32_t c01 = xo ? ldp(L.row, ix + 3 + iz) : ldp(L.row, 9 + 3 + iz);
+    brick = zo ? c01 : c00;
+  }
+  const uint32_t local = ((uint32_t)((L.x + ox) & 7) * 8u + (uint32_t)((L.y + oy) & 7)) * 8u + (uint32_t)((L.z + oz) & 7);
+  return (uint32_t)brick * (uint32_t)kBrickSites + local;
+}
 
 }  // namespace hlb

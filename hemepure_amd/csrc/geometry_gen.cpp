@@ -26,18 +26,7 @@
 #include "../../include/hemepure_amd.h"
 #include "d3q19.h"
 #include "errors.h"
-
-struct hlb_geom {
-  int32_t blocks[3] = {0, 0, 0};
-  int32_t bs = 8;
-  std::vector<int32_t> coords;
-  std::vector<uint8_t> link_type;
-  std::vector<int32_t> link_iolet;
-  std::vector<float> link_dist;
-  std::vector<uint8_t> normal_avail;
-  std::vector<float> normal;
-  std::vector<hlb_iolet_geom> iolets;
-};
+#include "geom.h"
 
 namespace {
 

@@ -16,16 +16,21 @@
 namespace hlb {
 
 hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, hipStream_t s);
+hipError_t launch_upstream_table(const BrickTables& bt, const uint32_t* bwall, const uint32_t* biolet, uint32_t* out,
+                                 int64_t S, hipStream_t s);
+size_t monitor_scratch(int64_t nslab);
+hipError_t launch_monitor(const BrickTables& bt, const double* rho, const double* u, int64_t S, void* scratch,
+                          double* out, hipStream_t s);
 hipError_t launch_gather_fp64(const StepArgs& a, double* F, int64_t S, hipStream_t s);
 hipError_t launch_fill_fp64(double* F, int64_t N, int64_t S, const double* vals, hipStream_t s);
-hipError_t launch_unstable_fp64(const double* F, int64_t N, int64_t S, int* flag, hipStream_t s);
+hipError_t launch_unstable_fp64(const BrickTables& bt, const double* F, int64_t S, int* flag, hipStream_t s);
 hipError_t launch_wk_begin(const WkRef* refs, int n, int step, hipStream_t s);
 hipError_t launch_wk_end(const WkRef* refs, int n, hipStream_t s);
 hipError_t launch_step_fp32(const Member32* members, const int32_t* wg_member, uint32_t grid, int step,
                             int variant, hipStream_t s);
-hipError_t launch_gather_fp32(const Member32* members, int64_t N, float* F, hipStream_t s);
+hipError_t launch_gather_fp32(const Member32* members, int64_t nslab, float* F, hipStream_t s);
 hipError_t launch_fill_fp32(float* F, int64_t N, int64_t S, const float* vals, hipStream_t s);
-hipError_t launch_unstable_fp32(const float* F, int64_t N, int64_t S, int* flag, hipStream_t s);
+hipError_t launch_unstable_fp32(const BrickTables& bt, const float* F, int64_t S, int* flag, hipStream_t s);
 hipError_t launch_cycle_delta(const double* u, float* prev, int64_t N, int64_t S, unsigned* out_bits,
                               hipStream_t s);
 
@@ -89,17 +94,24 @@ struct hlb_lattice {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // Device order (natural block order): perm[d] = reference index, inv[k] = d.
+  // Device slots (brick layout): perm[slot] = reference index or -1 for a
+  // hole, inv[k] = slot of reference index k.
   std::vector<int32_t> perm, inv;
+  int64_t nbrick = 0, nslab = 0;
+  std::vector<int32_t> h_nbr, h_slab, h_bbase;  // brick tables (host copies)
+  std::vector<uint64_t> h_fmask, h_bmask;
   // Boundary sites (classes 1-5) and iolet sites in device order.
   std::vector<int32_t> bsite;  // reference index of boundary slot b
   std::vector<int32_t> isite;  // reference index of iolet slot i
   int64_t S = 0, NB = 0, SB = 0, NI = 0, SI = 0;
   hlb::DevBuf<double> A[2], F, G, ladd_s, rho, u, wss, io_n, io_dens, io_vmax, feq_vals;
-  hlb::DevBuf<uint32_t> pull, bwall, biolet;
-  hlb::DevBuf<int32_t> bidx, bgi, bii;
+  hlb::DevBuf<uint32_t> bwall, biolet;
+  hlb::DevBuf<int32_t> bgi, bii, nbr, slab, bbase;
+  hlb::DevBuf<uint64_t> fmask, bmask;
   hlb::DevBuf<float> bq, bnrm, io_nf;
   hlb::DevBuf<int> flag;
+  hlb::DevBuf<char> mon_scratch;  // monitor partials, allocated on first use
+  hlb::DevBuf<double> mon_out;
   // FP32-shifted storage (params.precision == HLB_FP32_SHIFTED): float A, F,
   // G and the one-member table the member-table kernel reads.
   hlb::DevBuf<float> A32[2], F32, G32, feq32;
@@ -125,6 +137,7 @@ struct hlb_lattice {
   double omega = 0, half_omega = 0, half_omega_minus = 0, les_k = 0;
   std::vector<double> h_ladd_s;  // Ladd profile factors, 18 planes of SI, device iolet order
   bool device_ready = false;
+  bool iolets_set = false;   // hlb_create with params, or hlb_set_iolets
 
   ~hlb_lattice() {
     if (stream) {
@@ -137,6 +150,9 @@ struct hlb_lattice {
     if (h_mem) (void)hipHostFree(h_mem);
   }
   bool fp32() const { return params.precision == HLB_FP32_SHIFTED; }
+  hlb::BrickTables tables() const {
+    return hlb::BrickTables{nbr.p, slab.p, fmask.p, bmask.p, bbase.p, nslab};
+  }
 };
 
 namespace hlb {

@@ -116,23 +116,21 @@ __device__ __forceinline__ Planes member_planes(const Member32& m, int step) {
   return p;
 }
 
-__device__ __forceinline__ void gather_pull32(const Member32& m, const float* in, uint32_t d, uint32_t raw1,
-                                              float* g) {
-  uint32_t src[Q];
-  src[1] = raw1 & kIndexMask;
-#pragma unroll
-  for (int j = 2; j < Q; ++j) src[j] = ldp_nt(plane(m.pull, m.S, j - 1), d);
-  g[0] = ldp(in, d);
-#pragma unroll
-  for (int j = 1; j < Q; ++j) g[j] = ldp(plane(in, m.S, j), src[j]);
+template <int J = 1>
+__device__ __forceinline__ void gather_up32(const float* in, int64_t S, const Lane& L, uint32_t lm, float* g) {
+  if constexpr (J < Q) {
+    const uint32_t up = upstream_slot<J>(L);
+    const uint32_t src = (lm >> (INV[J] - 1)) & 1u ? L.slot : up;
+    g[J] = ldp(plane(in, S, J), src);
+    gather_up32<J + 1>(in, S, L, lm, g);
+  }
 }
 
 // The boundary delegates of one flagged site, as boundary_fixup in the FP64
 // pass (SimpleBounceBackDelegate.h:36-44, BouzidiFirdaousLallemandDelegate.h:
 // 44-93, iolet values from G).
 __device__ __forceinline__ void boundary_fixup32(const Member32& m, const float* in, uint32_t d, uint32_t b,
-                                                 float* g) {
-  const uint32_t wm = ldp(m.bwall, b), im = ldp(m.biolet, b);
+                                                 uint32_t wm, uint32_t im, float* g) {
   float own[Q];
 #pragma unroll
   for (int j = 1; j < Q; ++j) own[j] = ldp(plane(in, m.S, j), d);
@@ -229,7 +227,7 @@ __device__ __forceinline__ int member_of_tile(const int32_t* wg_member, uint32_t
   return wg_member ? __builtin_amdgcn_readfirstlane(ldp(wg_member, tile)) : 0;
 }
 
-// B sites per workgroup tile; REMAP: XCD-contiguous tile order.
+// B threads (B / 64 slabs) per workgroup tile; REMAP: XCD-contiguous tile order.
 template <int B, bool REMAP>
 __global__ __launch_bounds__(B) void step_fp32(const Member32* __restrict__ members,
                                                const int32_t* __restrict__ wg_member, int step) {
@@ -237,18 +235,25 @@ __global__ __launch_bounds__(B) void step_fp32(const Member32* __restrict__ memb
   const int mi = member_of_tile(wg_member, tile);
   if (mi < 0) return;  // padding tile
   const Member32& m = members[mi];
-  const uint32_t d = (tile - m.wg_begin) * B + threadIdx.x;
-  if (d >= (uint32_t)m.N) return;
+  const uint32_t r = (tile - m.wg_begin) * (B / kSlab) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  Lane L;
+  if (!slab_lane(m.bt, r, L)) return;
+  const uint32_t d = L.slot;
+  const bool isB = L.boundary;
+  uint32_t wm = 0, im = 0;
+  if (isB) {
+    wm = ldp(m.bwall, L.b);
+    im = ldp(m.biolet, L.b);
+  }
   const Planes P = member_planes(m, step);
-  const uint32_t raw1 = ldp_nt(m.pull, d);
-  const bool isB = (raw1 & kBoundaryFlag) != 0;
   float g[Q];
   if (P.first) {
 #pragma unroll
     for (int j = 0; j < Q; ++j) g[j] = ldp(plane(P.in, m.S, j), d);
   } else {
-    gather_pull32(m, P.in, d, raw1, g);
-    if (isB) boundary_fixup32(m, P.in, d, (uint32_t)ldp(m.bidx, d), g);
+    g[0] = ldp(P.in, d);
+    gather_up32(P.in, m.S, L, wm | im, g);
+    if (isB) boundary_fixup32(m, P.in, d, L.b, wm, im, g);
   }
 
   float drho, jx, jy, jz;
@@ -292,10 +297,9 @@ __global__ __launch_bounds__(B) void step_fp32(const Member32* __restrict__ memb
   const bool props = step == m.props_at;
   float w = FLT_MAX;
   if (isB) {
-    const uint32_t b = (uint32_t)ldp(m.bidx, d);
+    const uint32_t b = L.b;
     const int32_t ii = ldp(m.bii, b);
     if (ii >= 0) {
-      const uint32_t im = ldp(m.biolet, b);
       const int gi = ldp(m.bgi, b);
       const int bc = gi < m.n_in ? m.inlet_bc : m.outlet_bc;
       const double* dens = m.io_dens + (int64_t)step * m.nio;
@@ -307,7 +311,7 @@ __global__ __launch_bounds__(B) void step_fp32(const Member32* __restrict__ memb
         stp(m.wk_contrib, (uint32_t)ii, c);
       }
     }
-    if (props && ldp(m.bwall, b) != 0) w = wall_shear_stress32(m, b, tau, g, gpost);
+    if (props && wm != 0) w = wall_shear_stress32(m, b, tau, g, gpost);
   }
   if (props) {
     stp(m.rho, d, 1.0 + (double)drho);
@@ -320,17 +324,26 @@ __global__ __launch_bounds__(B) void step_fp32(const Member32* __restrict__ memb
 
 // Pre-collision g entering the next step into F (device order), for
 // downloads and the stability scan.
+__device__ __forceinline__ uint32_t slab_record4() {
+  return xcd_tile() * (kBlock / kSlab) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
+
 __global__ __launch_bounds__(kBlock) void gather_fp32(const Member32* __restrict__ members, float* __restrict__ F) {
   const Member32& m = members[0];
-  const uint32_t d = site_index();
-  if (d >= (uint32_t)m.N) return;
+  Lane L;
+  if (!slab_lane(m.bt, slab_record4(), L)) return;
   const float* in = m.A[m.cur0];
-  const uint32_t raw1 = ldp(m.pull, d);
+  uint32_t wm = 0, im = 0;
+  if (L.boundary) {
+    wm = ldp(m.bwall, L.b);
+    im = ldp(m.biolet, L.b);
+  }
   float g[Q];
-  gather_pull32(m, in, d, raw1, g);
-  if (raw1 & kBoundaryFlag) boundary_fixup32(m, in, d, (uint32_t)ldp(m.bidx, d), g);
+  g[0] = ldp(in, L.slot);
+  gather_up32(in, m.S, L, wm | im, g);
+  if (L.boundary) boundary_fixup32(m, in, L.slot, L.b, wm, im, g);
 #pragma unroll
-  for (int j = 0; j < Q; ++j) stp(plane(F, m.S, j), d, g[j]);
+  for (int j = 0; j < Q; ++j) stp(plane(F, m.S, j), L.slot, g[j]);
 }
 
 __global__ __launch_bounds__(kBlock) void fill_planes_fp32(float* __restrict__ F, int64_t N, int64_t S,
@@ -341,14 +354,14 @@ __global__ __launch_bounds__(kBlock) void fill_planes_fp32(float* __restrict__ F
   for (int j = 0; j < Q; ++j) F[(int64_t)j * S + k] = vals[j];
 }
 
-// StabilityTester on the shifted storage: f = g + w > 0.
-__global__ __launch_bounds__(kBlock) void unstable_fp32(const float* __restrict__ F, int64_t N, int64_t S,
+// StabilityTester on the shifted storage: f = g + w > 0 at every fluid site.
+__global__ __launch_bounds__(kBlock) void unstable_fp32(BrickTables bt, const float* __restrict__ F, int64_t S,
                                                         int* __restrict__ flag) {
-  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (k >= N) return;
+  Lane L;
+  if (!slab_lane(bt, slab_record4(), L)) return;
   bool bad = false;
 #pragma unroll
-  for (int j = 0; j < Q; ++j) bad |= !(F[(int64_t)j * S + k] > -wf(j));
+  for (int j = 0; j < Q; ++j) bad |= !(ldp(plane(F, S, j), L.slot) > -wf(j));
   if (bad) *flag = 1;
 }
 
@@ -387,6 +400,7 @@ hipError_t launch_cycle_delta(const double* u, float* prev, int64_t N, int64_t S
 }
 
 int fp32_variant_block(int variant) { return variant / 2 == 0 ? 128 : (variant / 2 == 1 ? 256 : 512); }
+int fp32_variant_slabs(int variant) { return fp32_variant_block(variant) / kSlab; }
 bool fp32_variant_remap(int variant) { return (variant & 1) != 0; }
 
 // `grid` workgroup tiles of the variant's size, padded to a multiple of the
@@ -407,10 +421,14 @@ hipError_t launch_step_fp32(const Member32* members, const int32_t* wg_member, u
   return hipGetLastError();
 }
 
-hipError_t launch_gather_fp32(const Member32* members, int64_t N, float* F, hipStream_t s) {
-  if (N == 0) return hipSuccess;
-  const uint32_t tiles = (uint32_t)((N + kBlock - 1) / kBlock);
-  const uint32_t grid = (tiles + kXcds - 1) / kXcds * kXcds;
+static uint32_t slab_grid4(int64_t nslab) {
+  const int64_t wg = (nslab + kBlock / kSlab - 1) / (kBlock / kSlab);
+  return (uint32_t)((wg + kXcds - 1) / kXcds * kXcds);
+}
+
+hipError_t launch_gather_fp32(const Member32* members, int64_t nslab, float* F, hipStream_t s) {
+  if (nslab == 0) return hipSuccess;
+  const uint32_t grid = slab_grid4(nslab);
   hipLaunchKernelGGL(gather_fp32, dim3(grid), dim3(kBlock), 0, s, members, F);
   return hipGetLastError();
 }
@@ -422,10 +440,9 @@ hipError_t launch_fill_fp32(float* F, int64_t N, int64_t S, const float* vals, h
   return hipGetLastError();
 }
 
-hipError_t launch_unstable_fp32(const float* F, int64_t N, int64_t S, int* flag, hipStream_t s) {
-  if (N == 0) return hipSuccess;
-  hipLaunchKernelGGL(unstable_fp32, dim3((unsigned)((N + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, F, N, S,
-                     flag);
+hipError_t launch_unstable_fp32(const BrickTables& bt, const float* F, int64_t S, int* flag, hipStream_t s) {
+  if (bt.nslab == 0) return hipSuccess;
+  hipLaunchKernelGGL(unstable_fp32, dim3(slab_grid4(bt.nslab)), dim3(kBlock), 0, s, bt, F, S, flag);
   return hipGetLastError();
 }
 

@@ -20,6 +20,7 @@
 // neighbours also read stay in that XCD's L2.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 
 #include "../../include/hemepure_amd.h"
@@ -31,15 +32,22 @@ namespace hlb {
 
 namespace {
 
-// Gathers along the pull table; raw1 is plane 0 with the boundary flag.
-__device__ __forceinline__ void gather_pull(const StepArgs& a, uint32_t d, uint32_t raw1, double* f) {
-  uint32_t src[Q];
-  src[1] = raw1 & kIndexMask;
-#pragma unroll
-  for (int j = 2; j < Q; ++j) src[j] = ldp_nt(a.pullp[j - 1], d);
-  f[0] = ldp(a.ain[0], d);
-#pragma unroll
-  for (int j = 1; j < Q; ++j) f[j] = ldp(a.ain[j], src[j]);
+// Gathers the upstream post-collision values.  lm marks the directions
+// whose link ends on a wall or iolet (boundary lanes): those read the lane's
+// own slot, which the boundary delegates then replace.
+template <int J = 1>
+__device__ __forceinline__ void gather_up(const double* const* ain, const Lane& L, uint32_t lm, double* f) {
+  if constexpr (J < Q) {
+    const uint32_t up = upstream_slot<J>(L);
+    const uint32_t src = (lm >> (INV[J] - 1)) & 1u ? L.slot : up;
+    f[J] = ldp(ain[J], src);
+    gather_up<J + 1>(ain, L, lm, f);
+  }
+}
+
+__device__ __forceinline__ void gather_site(const StepArgs& a, const Lane& L, uint32_t wm, uint32_t im, double* f) {
+  f[0] = ldp(a.ain[0], L.slot);
+  gather_up(a.ain, L, wm | im, f);
 }
 
 // Boundary sites: per direction j the writer of the value is the site's link
@@ -47,8 +55,8 @@ __device__ __forceinline__ void gather_pull(const StepArgs& a, uint32_t d, uint3
 // own post-collision values) or the neighbour (already gathered).  The own
 // row and the link distances are loaded before any selection so a wave waits
 // on one round of loads.
-__device__ __forceinline__ void boundary_fixup(const StepArgs& a, uint32_t d, uint32_t b, double* f) {
-  const uint32_t wm = ldp(a.bwall, b), im = ldp(a.biolet, b);
+__device__ __forceinline__ void boundary_fixup(const StepArgs& a, uint32_t d, uint32_t b, uint32_t wm,
+                                               uint32_t im, double* f) {
   double own[Q];
 #pragma unroll
   for (int j = 1; j < Q; ++j) own[j] = ldp(a.ain[j], d);
@@ -155,20 +163,30 @@ __device__ __forceinline__ double wall_shear_stress(const StepArgs& a, uint32_t 
   return __builtin_sqrt(sq - nn * nn);
 }
 
+// The slab record of this lane's wavefront: four slabs per workgroup.
+__device__ __forceinline__ uint32_t slab_record() {
+  return xcd_tile() * (kBlock / kSlab) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
+
 template <int KERNEL, bool FIRST, bool PROPS>
 __global__ __launch_bounds__(kBlock) void step_fp64(StepArgs a) {
-  const uint32_t d = site_index();
-  if (d >= (uint32_t)a.N) return;
-  const uint32_t raw1 = ldp_nt(a.pullp[0], d);
-  const bool isB = (raw1 & kBoundaryFlag) != 0;
+  Lane L;
+  if (!slab_lane(a.bt, slab_record(), L)) return;
+  const uint32_t d = L.slot;
+  const bool isB = L.boundary;
+  uint32_t wm = 0, im = 0;
+  if (isB) {
+    wm = ldp(a.bwall, L.b);
+    im = ldp(a.biolet, L.b);
+  }
   double f[Q];
   if constexpr (FIRST) {
     // Initial condition or checkpoint: f is the pre-collision state itself.
 #pragma unroll
     for (int j = 0; j < Q; ++j) f[j] = ldp(a.ain[j], d);
   } else {
-    gather_pull(a, d, raw1, f);
-    if (isB) boundary_fixup(a, d, (uint32_t)ldp(a.bidx, d), f);
+    gather_site(a, L, wm, im, f);
+    if (isB) boundary_fixup(a, d, L.b, wm, im, f);
   }
 
   // Moments and equilibrium (Lattice.h:933-948), collision.
@@ -222,10 +240,9 @@ __global__ __launch_bounds__(kBlock) void step_fp64(StepArgs a) {
 
   double w = DBL_MAX;
   if (isB) {
-    const uint32_t b = (uint32_t)ldp(a.bidx, d);
+    const uint32_t b = L.b;
     const int32_t ii = ldp(a.bii, b);
     if (ii >= 0) {
-      const uint32_t im = ldp(a.biolet, b);
       const int gi = ldp(a.bgi, b);
       const int bc = gi < a.n_in ? a.inlet_bc : a.outlet_bc;
       iolet_links(a, (uint32_t)ii, im, gi, bc, rho, ux, uy, uz, fpost);
@@ -236,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void step_fp64(StepArgs a) {
       }
     }
     if constexpr (PROPS) {
-      if (ldp(a.bwall, b) != 0) w = wall_shear_stress(a, b, tau, f, fpost);
+      if (wm != 0) w = wall_shear_stress(a, b, tau, f, fpost);
     }
   }
   if constexpr (PROPS) {
@@ -252,14 +269,103 @@ __global__ __launch_bounds__(kBlock) void step_fp64(StepArgs a) {
 // Materialises the pre-collision f (the reference's fOld entering the next
 // step) into F, device order.
 __global__ __launch_bounds__(kBlock) void gather_fp64(StepArgs a, double* __restrict__ F, int64_t S) {
-  const uint32_t d = site_index();
-  if (d >= (uint32_t)a.N) return;
-  const uint32_t raw1 = ldp(a.pullp[0], d);
+  Lane L;
+  if (!slab_lane(a.bt, slab_record(), L)) return;
+  uint32_t wm = 0, im = 0;
+  if (L.boundary) {
+    wm = ldp(a.bwall, L.b);
+    im = ldp(a.biolet, L.b);
+  }
   double f[Q];
-  gather_pull(a, d, raw1, f);
-  if (raw1 & kBoundaryFlag) boundary_fixup(a, d, (uint32_t)ldp(a.bidx, d), f);
+  gather_site(a, L, wm, im, f);
+  if (L.boundary) boundary_fixup(a, L.slot, L.b, wm, im, f);
 #pragma unroll
-  for (int j = 0; j < Q; ++j) stp(plane(F, S, j), d, f[j]);
+  for (int j = 0; j < Q; ++j) stp(plane(F, S, j), L.slot, f[j]);
+}
+
+// The slot each lane gathers direction J from (the lane's own slot for a
+// wall or iolet link), 18 planes of S: the step kernels' addressing, exported
+// so the host can compare it with the reference's neighbour table.
+template <int J = 1>
+__device__ __forceinline__ void upstream_all(const Lane& L, uint32_t lm, uint32_t* out, int64_t S) {
+  if constexpr (J < Q) {
+    const uint32_t up = upstream_slot<J>(L);
+    stp(plane(out, S, J - 1), L.slot, (lm >> (INV[J] - 1)) & 1u ? L.slot : up);
+    upstream_all<J + 1>(L, lm, out, S);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void upstream_table(BrickTables bt, const uint32_t* __restrict__ bwall,
+                                                         const uint32_t* __restrict__ biolet,
+                                                         uint32_t* __restrict__ out, int64_t S) {
+  Lane L;
+  if (!slab_lane(bt, slab_record(), L)) return;
+  const uint32_t lm = L.boundary ? (ldp(bwall, L.b) | ldp(biolet, L.b)) : 0u;
+  upstream_all(L, lm, out, S);
+}
+
+// Monitors over the fluid sites' property fields (the last step that filled
+// them): min rho, max rho, max |u| -- the IncompressibilityChecker's
+// DensityTracker (IncompressibilityChecker.hpp:170-195, |u| as
+// Vector3D::GetMagnitude) -- and the total mass sum rho.  Pass 1 reduces each
+// workgroup in a fixed tree order into partial[wg]; pass 2 reduces the
+// partials in order, so the sum is reproducible run to run.
+struct Mon {
+  double lo, hi, umax, mass;
+};
+
+__device__ __forceinline__ Mon mon_join(Mon a, Mon b) {
+  return Mon{__builtin_fmin(a.lo, b.lo), __builtin_fmax(a.hi, b.hi), __builtin_fmax(a.umax, b.umax), a.mass + b.mass};
+}
+
+__device__ __forceinline__ Mon mon_wave(Mon m) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    Mon o{__shfl_xor(m.lo, off, 64), __shfl_xor(m.hi, off, 64), __shfl_xor(m.umax, off, 64),
+          __shfl_xor(m.mass, off, 64)};
+    m = mon_join(m, o);
+  }
+  return m;
+}
+
+__global__ __launch_bounds__(kBlock) void monitor_pass1(BrickTables bt, const double* __restrict__ rho,
+                                                        const double* __restrict__ u, int64_t S,
+                                                        Mon* __restrict__ partial) {
+  __shared__ Mon w[kBlock / 64];
+  Mon m{DBL_MAX, -DBL_MAX, 0.0, 0.0};
+  Lane L;
+  if (slab_lane(bt, slab_record(), L)) {
+    const double r = ldp(rho, L.slot);
+    const double x = ldp(u, L.slot), y = ldp(plane(u, S, 1), L.slot), z = ldp(plane(u, S, 2), L.slot);
+    m = Mon{r, r, __builtin_sqrt(x * x + y * y + z * z), r};
+  }
+  m = mon_wave(m);
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Mon t = w[0];
+    for (int i = 1; i < kBlock / 64; ++i) t = mon_join(t, w[i]);
+    partial[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void monitor_pass2(const Mon* __restrict__ partial, int n, double* __restrict__ out) {
+  __shared__ Mon w[kBlock / 64];
+  Mon m{DBL_MAX, -DBL_MAX, 0.0, 0.0};
+  // each lane folds a contiguous run, in order
+  const int per = (n + kBlock - 1) / kBlock;
+  for (int i = threadIdx.x * per; i < n && i < (int)(threadIdx.x + 1) * per; ++i) m = mon_join(m, partial[i]);
+  m = mon_wave(m);
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Mon t = w[0];
+    for (int i = 1; i < kBlock / 64; ++i) t = mon_join(t, w[i]);
+    out[0] = t.lo;
+    out[1] = t.hi;
+    out[2] = t.umax;
+    out[3] = t.mass;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void fill_planes_fp64(double* __restrict__ F, int64_t N, int64_t S,
@@ -270,14 +376,14 @@ __global__ __launch_bounds__(kBlock) void fill_planes_fp64(double* __restrict__ 
   for (int j = 0; j < Q; ++j) F[(int64_t)j * S + k] = vals[j];
 }
 
-// StabilityTester: flag any !(f > 0) among the sites' distributions.
-__global__ __launch_bounds__(kBlock) void unstable_fp64(const double* __restrict__ F, int64_t N, int64_t S,
+// StabilityTester: flag any !(f > 0) among the fluid sites' distributions.
+__global__ __launch_bounds__(kBlock) void unstable_fp64(BrickTables bt, const double* __restrict__ F, int64_t S,
                                                         int* __restrict__ flag) {
-  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (k >= N) return;
+  Lane L;
+  if (!slab_lane(bt, slab_record(), L)) return;
   bool bad = false;
 #pragma unroll
-  for (int j = 0; j < Q; ++j) bad |= !(F[(int64_t)j * S + k] > 0.0);
+  for (int j = 0; j < Q; ++j) bad |= !(ldp(plane(F, S, j), L.slot) > 0.0);
   if (bad) *flag = 1;
 }
 
@@ -361,11 +467,15 @@ hipError_t launch_wk_end(const WkRef* refs, int n, hipStream_t s) {
 unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 // Site grids are padded to a multiple of the XCD count for the remapping.
-static unsigned site_grid(int64_t n) { return (grid_for(n) + kXcds - 1) / kXcds * kXcds; }
+// Workgroups of four slabs, padded to a multiple of the XCD count.
+static unsigned slab_grid(int64_t nslab) {
+  const int64_t wg = (nslab + kBlock / kSlab - 1) / (kBlock / kSlab);
+  return (unsigned)((wg + kXcds - 1) / kXcds * kXcds);
+}
 
 hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, hipStream_t s) {
-  if (a.N <= 0) return hipSuccess;
-  const dim3 grid(site_grid(a.N)), block(kBlock);
+  if (a.bt.nslab <= 0) return hipSuccess;
+  const dim3 grid(slab_grid(a.bt.nslab)), block(kBlock);
 #define HLB_LAUNCH(KN, FI, PR) hipLaunchKernelGGL((step_fp64<KN, FI, PR>), grid, block, 0, s, a)
 #define HLB_KSEL(KN)                                   \
   if (first) {                                         \
@@ -385,9 +495,28 @@ hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool prop
   return hipGetLastError();
 }
 
+hipError_t launch_upstream_table(const BrickTables& bt, const uint32_t* bwall, const uint32_t* biolet, uint32_t* out,
+                                 int64_t S, hipStream_t s) {
+  if (bt.nslab <= 0) return hipSuccess;
+  hipLaunchKernelGGL(upstream_table, dim3(slab_grid(bt.nslab)), dim3(kBlock), 0, s, bt, bwall, biolet, out, S);
+  return hipGetLastError();
+}
+
+size_t monitor_scratch(int64_t nslab) { return sizeof(Mon) * std::max<size_t>(1, slab_grid(nslab)); }
+
+hipError_t launch_monitor(const BrickTables& bt, const double* rho, const double* u, int64_t S, void* scratch,
+                          double* out, hipStream_t s) {
+  if (bt.nslab <= 0) return hipSuccess;
+  const unsigned grid = slab_grid(bt.nslab);
+  Mon* part = static_cast<Mon*>(scratch);
+  hipLaunchKernelGGL(monitor_pass1, dim3(grid), dim3(kBlock), 0, s, bt, rho, u, S, part);
+  hipLaunchKernelGGL(monitor_pass2, dim3(1), dim3(kBlock), 0, s, part, (int)grid, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_gather_fp64(const StepArgs& a, double* F, int64_t S, hipStream_t s) {
-  if (a.N <= 0) return hipSuccess;
-  hipLaunchKernelGGL(gather_fp64, dim3(site_grid(a.N)), dim3(kBlock), 0, s, a, F, S);
+  if (a.bt.nslab <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_fp64, dim3(slab_grid(a.bt.nslab)), dim3(kBlock), 0, s, a, F, S);
   return hipGetLastError();
 }
 
@@ -397,9 +526,9 @@ hipError_t launch_fill_fp64(double* F, int64_t N, int64_t S, const double* vals,
   return hipGetLastError();
 }
 
-hipError_t launch_unstable_fp64(const double* F, int64_t N, int64_t S, int* flag, hipStream_t s) {
-  if (N <= 0) return hipSuccess;
-  hipLaunchKernelGGL(unstable_fp64, dim3(grid_for(N)), dim3(kBlock), 0, s, F, N, S, flag);
+hipError_t launch_unstable_fp64(const BrickTables& bt, const double* F, int64_t S, int* flag, hipStream_t s) {
+  if (bt.nslab <= 0) return hipSuccess;
+  hipLaunchKernelGGL(unstable_fp64, dim3(slab_grid(bt.nslab)), dim3(kBlock), 0, s, bt, F, S, flag);
   return hipGetLastError();
 }
 

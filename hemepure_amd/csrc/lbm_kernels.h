@@ -1,27 +1,34 @@
 // lbm_kernels.h -- device-side data layout shared by the kernels and the
 // host launcher.
 //
-// Device order.  The reference's LatticeData orders sites class-major
+// Brick layout.  The reference's LatticeData orders sites class-major
 // (LatticeData.h:375-436); the host keeps that order for every API array.  On
-// the device the N sites are permuted into natural block order -- blocks in
-// the reference's block order, sites in a block z-fastest, all collision
-// classes together -- so that a wavefront's 64 sites are a compact 8x8 slab
-// and the boundary sites are processed next to the neighbours they exchange
-// values with.  perm[d] is the reference index of device slot d.
+// the device every occupied 8^3 block of the geometry is a brick of 512
+// slots, bricks in the reference's block order, slot = brick * 512 +
+// (x * 8 + y) * 8 + z with block-local coordinates (the reference's in-block
+// order, z fastest).  Solid positions are holes that no kernel writes.  One
+// wavefront handles one slab (the 64 slots of one x) and a launch walks the
+// list of slabs that hold at least one fluid site.  A site's upstream
+// neighbour is found arithmetically: the same brick, or one of its 26
+// neighbouring bricks from a 27-entry row per brick, so no per-site
+// neighbour table is read.  Links that end on a wall or iolet read the site's
+// own slot instead and the boundary delegates overwrite the value.
 //
-// HBM layout (device order, S = N rounded up to 64):
-//   A[2]   post-collision distributions, 19 planes of S doubles.
-//   F      pre-collision staging, same shape (initial state, uploads and
-//          downloads; allocated on demand).
-//   pull   18 planes of S uint32: pull[j-1][d] is the device slot whose
-//          post-collision value of direction j arrives at d, or d itself
-//          where a boundary condition supplies the value.  Bit 31 of plane 0
-//          flags the sites of collision classes 1-5 (wall or iolet links).
-//   bidx   S int32: compact boundary index of a flagged site.
-//   boundary arrays (NB flagged sites in device order, stride SB): wall and
-//          iolet link masks, global iolet index, compact iolet index, 18
-//          planes of float link distances, 3 planes of float wall normals.
-//   iolet arrays (NI sites with iolet links, device order, stride SI): G, 19
+// HBM layout (S = (bricks + 1) * 512 slots; the extra brick is the empty
+// neighbour of bricks at the edge of the geometry):
+//   A[2]   post-collision distributions, 19 planes of S (double, or float
+//          g = f - w rho0 in the FP32-shifted form).
+//   F      pre-collision staging of the same shape (initial state, uploads,
+//          downloads).
+//   nbr    (bricks + 1) x 27 brick ids, index (dx+1)*9 + (dy+1)*3 + (dz+1).
+//   slabs  per active slab: brick * 8 + x, the fluid-lane and boundary-lane
+//          masks (lane = y * 8 + z) and the compact index of its first
+//          boundary lane; boundary sites (collision classes 1-5) are numbered
+//          in slot order.
+//   boundary arrays (NB boundary sites, stride SB): wall and iolet link
+//          masks, global iolet index, compact iolet index, 18 planes of float
+//          link distances, 3 planes of float wall normals.
+//   iolet arrays (NI sites with iolet links, slot order, stride SI): G, 19
 //          planes of the values the site's iolet delegates computed for the
 //          next step, and 18 planes of Ladd profile factors.
 #pragma once
@@ -30,15 +37,27 @@
 
 namespace hlb {
 
-constexpr uint32_t kBoundaryFlag = 0x80000000u;
-constexpr uint32_t kIndexMask = 0x7fffffffu;
+constexpr int kBrickSites = 512;  // slots per brick (8^3)
+constexpr int kSlab = 64;         // slots per slab = lanes per wavefront
+
+// The brick tables of one lattice.
+struct BrickTables {
+  const int32_t* nbr;    // (bricks + 1) x 27
+  const int32_t* slab;   // per active slab: brick * 8 + x
+  const uint64_t* fmask; // fluid lanes
+  const uint64_t* bmask; // boundary lanes
+  const int32_t* bbase;  // compact boundary index of the slab's first boundary lane
+  int64_t nslab;         // active slabs
+};
 constexpr int kMaxIolets = 256;  // iolets per lattice
 
-// Launch shapes of the FP32 member-table kernel: workgroup tile of 128, 256
-// or 512 sites, each with and without the XCD-contiguous tile order.
+// Launch shapes of the FP32 member-table kernel: workgroup tile of 2, 4 or 8
+// slabs (128, 256, 512 threads), each with and without the XCD-contiguous
+// tile order.
 constexpr int kFp32Variants = 6;
 constexpr int kFp32DefaultVariant = 3;  // 256 sites, XCD order
 int fp32_variant_block(int variant);
+int fp32_variant_slabs(int variant);  // slabs per workgroup tile
 bool fp32_variant_remap(int variant);
 
 struct StepArgs {
@@ -46,8 +65,7 @@ struct StepArgs {
   // SGPR base plus a 32-bit VGPR offset.
   const double* ain[19];
   double* aout[19];
-  const uint32_t* pullp[18];
-  const int32_t* __restrict__ bidx;
+  BrickTables bt;
   const uint32_t* __restrict__ bwall;
   const uint32_t* __restrict__ biolet;
   const int32_t* __restrict__ bgi;  // global iolet index: inlets 0..n_in-1, outlets follow
@@ -85,8 +103,7 @@ struct StepArgs {
 struct Member32 {
   float* A[2];            // post-collision g, 19 planes of S
   const float* F;         // pre-collision g (initial state / checkpoint)
-  const uint32_t* pull;   // 18 planes of S, boundary flag on plane 0
-  const int32_t* bidx;
+  BrickTables bt;
   const uint32_t* bwall;
   const uint32_t* biolet;
   const int32_t* bgi;
@@ -110,7 +127,7 @@ struct Member32 {
   int32_t cur0;           // A[cur0] holds the latest values when the sequence starts
   int32_t first0;         // 1: step 0 of the sequence reads F (pre-collision)
   int32_t props_at;       // step of the sequence that writes rho, u, WSS (-1: none)
-  uint32_t wg_begin;      // first workgroup tile of this member in the launch
+  uint32_t wg_begin;      // first workgroup tile of this member in the launch (tiles of whole slabs)
   float tau, omega, half_omega, half_omega_minus, les_k;
   int32_t pad_;
 };

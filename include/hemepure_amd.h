@@ -134,6 +134,18 @@ const char* hlb_last_error(void);
  * distribution in the reference's push step). */
 int hlb_create(const hlb_geometry* geom, const hlb_params* params, const hlb_iolet_desc* inlets,
                int n_inlets, const hlb_iolet_desc* outlets, int n_outlets, hlb_handle* out);
+/* With params == NULL hlb_create builds only the lattice tables (the
+ * reference's LatticeData before an LBM exists); the collision/boundary
+ * selection and the iolets then come from hlb_set_params and hlb_set_iolets.
+ * hlb_set_params replaces LBM::ReadParameters / LbmParameters::Update
+ * (src/lb/lb.hpp:361-374, src/lb/LbmParameters.h:40-44): once the lattice has
+ * a device state only tau and trt_lambda may change.  hlb_set_iolets replaces
+ * BoundaryValues::BoundaryValues (src/lb/iolets/BoundaryValues.cc:20-81) and
+ * is allowed until the lattice has a device state. */
+int hlb_set_params(hlb_handle h, const hlb_params* params);
+int hlb_get_params(hlb_handle h, hlb_params* out);
+int hlb_set_iolets(hlb_handle h, const hlb_iolet_desc* inlets, int n_inlets, const hlb_iolet_desc* outlets,
+                   int n_outlets);
 /* Replaces LBM::~LBM / LatticeData::~LatticeData. */
 void hlb_destroy(hlb_handle h);
 
@@ -264,6 +276,14 @@ int64_t hlb_server_bytes_in_use(hlb_server_handle s);
 /* Device bytes a lattice holds once it has a state (admission accounting). */
 int64_t hlb_device_bytes(hlb_handle h);
 
+/* Monitors from the fields of the last step run with HLB_STEP_PROPERTIES,
+ * reduced on the device over the fluid sites: out[0] smallest density,
+ * out[1] largest density, out[2] largest |u| (the DensityTracker of
+ * IncompressibilityChecker, src/lb/IncompressibilityChecker.hpp:170-195;
+ * max relative density difference = (out[1] - out[0]) / 1), out[3] total
+ * mass sum rho in FP64, summed in a fixed order (reproducible). */
+int hlb_monitor(hlb_handle h, double out[4]);
+
 /* Iolet state (outlet = 0: inlet `index`, 1: outlet `index`):
  * out[0] density used by the last step's ghost sites (for Windkessel iolets,
  * the committed density), out[1] densityNew, out[2] flowRate, out[3]
@@ -312,6 +332,12 @@ int hlb_geom_ybifurcation(double rp, double lp, double rd, double ld, double ang
  * mid-length, width parameter w (lattice units). */
 int hlb_geom_stenosed_pipe(double radius, int32_t length, double area_reduction, double width,
                            hlb_geom** out);
+/* Reads a HemeLB geometry file, version 4 (gmy_plus = 1: the .gmy+ variant
+ * with per-block weights); replaces GeometryReader::LoadAndDecompose's
+ * reading (src/geometry/GeometryReader.cc:382-446, 851-935) on one process.
+ * The file carries no iolet positions (they come from the XML): the returned
+ * geometry lists none. */
+int hlb_geom_read_gmy(const char* path, int gmy_plus, hlb_geom** out);
 /* Fills the view with pointers into g (valid until hlb_geom_free). */
 int hlb_geom_view(const hlb_geom* g, hlb_geometry* view);
 int hlb_geom_iolets(const hlb_geom* g, hlb_iolet_geom* out, int max_iolets);

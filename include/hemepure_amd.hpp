@@ -1,0 +1,473 @@
+/*
+ * hemepure_amd.hpp -- the reference's C++ host API, over the C ABI.
+ *
+ * HemePure's host code drives its hot path through SimulationMaster,
+ * geometry::LatticeData and lb::LBM<Lattice> with the collision and
+ * boundary types chosen at build time (src/lb/BuildSystemInterface.h:24-369,
+ * consumed at src/lb/lb.h:37-48).  This header keeps those names and that
+ * shape so reference-side code reads the same, and routes every call to the
+ * MI355X library (hemepure_amd.h).  Kernel and streamer selection stays a
+ * compile-time choice: the tag types below carry the library's enum values.
+ * Header-only; link with -lhemepure_amd.
+ */
+#ifndef HEMEPURE_AMD_HPP
+#define HEMEPURE_AMD_HPP
+
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "hemepure_amd.h"
+
+namespace hemelb {
+
+typedef int64_t site_t;
+typedef double distribn_t;
+typedef double LatticeDensity;
+typedef double PhysicalTime;
+typedef double PhysicalDistance;
+
+/* hemelb::Exception (src/Exception.h:25): thrown with the library's message. */
+class Exception : public std::runtime_error {
+ public:
+  explicit Exception(const std::string& what) : std::runtime_error(what) {}
+};
+
+namespace detail {
+inline void Check(int rc) {
+  if (rc < 0) throw Exception(hlb_last_error());
+}
+struct HandleDeleter {
+  void operator()(hlb_lattice* h) const { hlb_destroy(h); }
+};
+struct GeomDeleter {
+  void operator()(hlb_geom* g) const { hlb_geom_free(g); }
+};
+}  // namespace detail
+
+namespace util {
+template <class T>
+struct Vector3D {
+  T x, y, z;
+  T operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+}  // namespace util
+
+namespace geometry {
+
+/* The decoded content of a geometry file (GeometryReader on one process), or
+ * a generated vessel. */
+class Geometry {
+ public:
+  static Geometry Read(const std::string& path, bool gmyPlus = false) {
+    hlb_geom* g = nullptr;
+    detail::Check(hlb_geom_read_gmy(path.c_str(), gmyPlus ? 1 : 0, &g));
+    return Geometry(g);
+  }
+  static Geometry Cylinder(double radius, int32_t length, int32_t boxXY) {
+    hlb_geom* g = nullptr;
+    detail::Check(hlb_geom_cylinder(radius, length, boxXY, &g));
+    return Geometry(g);
+  }
+  static Geometry StenosedPipe(double radius, int32_t length, double areaReduction, double width) {
+    hlb_geom* g = nullptr;
+    detail::Check(hlb_geom_stenosed_pipe(radius, length, areaReduction, width, &g));
+    return Geometry(g);
+  }
+  hlb_geometry View() const {
+    hlb_geometry v;
+    detail::Check(hlb_geom_view(geom.get(), &v));
+    return v;
+  }
+  /* Iolet positions, normals and radii a generator knows (none for a file). */
+  std::vector<hlb_iolet_geom> Iolets() const {
+    std::vector<hlb_iolet_geom> v(64);
+    const int n = hlb_geom_iolets(geom.get(), v.data(), (int)v.size());
+    detail::Check(n);
+    v.resize((size_t)n);
+    return v;
+  }
+
+ private:
+  explicit Geometry(hlb_geom* g) : geom(g, detail::GeomDeleter()) {}
+  std::shared_ptr<hlb_geom> geom;
+};
+
+/* LatticeData (src/geometry/LatticeData.h:59-520): the site classification,
+ * the class-major site order and the neighbour tables, on one process (so
+ * every fluid site is mid-domain).  The handle it owns becomes the LBM's. */
+class LatticeData {
+ public:
+  explicit LatticeData(const Geometry& g) : geometry(g) {
+    const hlb_geometry v = g.View();
+    hlb_handle h = nullptr;
+    detail::Check(hlb_create(&v, nullptr, nullptr, 0, nullptr, 0, &h));
+    handle.reset(h, detail::HandleDeleter());
+    siteCount = hlb_site_count(h);
+    counts.resize(6);
+    detail::Check(hlb_class_counts(h, counts.data()));
+  }
+  site_t GetLocalFluidSiteCount() const { return siteCount; }
+  site_t GetMidDomainCollisionCount(unsigned collisionType) const { return counts.at(collisionType); }
+  site_t GetDomainEdgeCollisionCount(unsigned) const { return 0; }
+  util::Vector3D<site_t> GetGlobalSiteCoords(site_t i) const {
+    Coords();
+    return util::Vector3D<site_t>{coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
+  }
+  /* LatticeData::GetStreamedIndex (LatticeData.h:516-520): the push target
+   * of (site, direction), rubbish slot N * 19 when the neighbour is solid. */
+  site_t GetStreamedIndex(site_t i, unsigned direction) const {
+    Tables();
+    return push[(size_t)i * 19 + direction];
+  }
+  bool HasWall(site_t i, unsigned direction) const {
+    Masks();
+    return direction > 0 && ((wall[i] >> (direction - 1)) & 1u);
+  }
+  bool HasIolet(site_t i, unsigned direction) const {
+    Masks();
+    return direction > 0 && ((iolet[i] >> (direction - 1)) & 1u);
+  }
+  int GetIoletId(site_t i) const {
+    Masks();
+    return ioletId[i];
+  }
+  int GetSiteType(site_t i) const {
+    Masks();
+    return siteType[i];
+  }
+  /* SiteData::GetCollisionType order: 0 bulk .. 5 outlet + wall. */
+  unsigned GetCollisionType(site_t i) const {
+    site_t end = 0;
+    for (unsigned c = 0; c < 6; ++c) {
+      end += counts[c];
+      if (i < end) return c;
+    }
+    throw Exception("LatticeData::GetCollisionType: no such site");
+  }
+  hlb_handle Handle() const { return handle.get(); }
+
+ private:
+  void Coords() const {
+    if (!coords.empty()) return;
+    coords.resize((size_t)siteCount * 3);
+    detail::Check(hlb_site_coords(handle.get(), coords.data()));
+  }
+  void Tables() const {
+    if (!push.empty()) return;
+    push.resize((size_t)siteCount * 19);
+    detail::Check(hlb_neighbour_tables(handle.get(), push.data(), nullptr));
+  }
+  void Masks() const {
+    if (!wall.empty()) return;
+    wall.resize((size_t)siteCount);
+    iolet.resize((size_t)siteCount);
+    ioletId.resize((size_t)siteCount);
+    siteType.resize((size_t)siteCount);
+    detail::Check(hlb_site_masks(handle.get(), wall.data(), iolet.data(), ioletId.data(), siteType.data()));
+  }
+  Geometry geometry;
+  std::shared_ptr<hlb_lattice> handle;
+  site_t siteCount = 0;
+  std::vector<int64_t> counts;
+  mutable std::vector<int64_t> coords, push;
+  mutable std::vector<uint32_t> wall, iolet;
+  mutable std::vector<int32_t> ioletId, siteType;
+};
+
+}  // namespace geometry
+
+namespace lb {
+
+namespace lattices {
+/* D3Q19 (src/lb/lattices/D3Q19.cc:15-33). */
+struct D3Q19 {
+  static constexpr unsigned NUMVECTORS = 19;
+};
+}  // namespace lattices
+
+/* The build-time selections of BuildSystemInterface.h as tag types. */
+namespace kernels {
+template <class Lattice>
+struct LBGK {
+  static constexpr int32_t Code = HLB_KERNEL_LBGK;
+};
+template <class Lattice>
+struct TRT {
+  static constexpr int32_t Code = HLB_KERNEL_TRT;
+};
+/* LBGKLESSpongeLayer with sponge lifetime 0 (pure Smagorinsky, C_s = 0.1). */
+template <class Lattice>
+struct LBGKLESSL {
+  static constexpr int32_t Code = HLB_KERNEL_LES;
+};
+}  // namespace kernels
+
+namespace streamers {
+template <class Collision>
+struct SimpleBounceBack {
+  static constexpr int32_t Wall = HLB_WALL_SBB;
+};
+template <class Collision>
+struct BouzidiFirdaousLallemand {
+  static constexpr int32_t Wall = HLB_WALL_BFL;
+};
+template <class Collision>
+struct NashZerothOrderPressureIolet {
+  static constexpr int32_t Iolet = HLB_IOLET_NASH;
+};
+template <class Collision>
+struct LaddIolet {
+  static constexpr int32_t Iolet = HLB_IOLET_LADD;
+};
+}  // namespace streamers
+
+/* Iolet value providers (src/lb/iolets/), in lattice units. */
+namespace iolets {
+inline hlb_iolet_desc Base(int32_t kind, const double position[3], const double normal[3]) {
+  hlb_iolet_desc d{};
+  d.kind = kind;
+  for (int a = 0; a < 3; ++a) {
+    d.position[a] = position[a];
+    d.normal[a] = normal[a];
+  }
+  d.density_mean = 1.0;
+  d.period = 1.0;
+  d.area = d.resistance = d.r_c = d.r_p = d.capacitance = 1.0;
+  return d;
+}
+inline hlb_iolet_desc InOutLetCosine(const double position[3], const double normal[3], LatticeDensity mean,
+                                     LatticeDensity amplitude = 0.0, double phase = 0.0, double period = 1.0,
+                                     int32_t warmup = 0) {
+  hlb_iolet_desc d = Base(HLB_IOLET_COSINE, position, normal);
+  d.density_mean = mean;
+  d.density_amp = amplitude;
+  d.phase = phase;
+  d.period = period;
+  d.warmup = warmup;
+  return d;
+}
+inline hlb_iolet_desc InOutLetParabolicVelocity(const double position[3], const double normal[3], double radius,
+                                                double maxSpeed, int32_t warmup = 0) {
+  hlb_iolet_desc d = Base(HLB_IOLET_PARABOLIC, position, normal);
+  d.radius = radius;
+  d.max_speed = maxSpeed;
+  d.warmup = warmup;
+  return d;
+}
+/* The table (max speed per 1-indexed step) is copied when the LBM takes it. */
+inline hlb_iolet_desc InOutLetFileVelocity(const double position[3], const double normal[3], double radius,
+                                           const std::vector<double>& table) {
+  hlb_iolet_desc d = Base(HLB_IOLET_FILEVEL, position, normal);
+  d.radius = radius;
+  d.table = table.data();
+  d.table_len = (int64_t)table.size();
+  return d;
+}
+inline hlb_iolet_desc InOutLetWK2(const double position[3], const double normal[3], double radius, double area,
+                                  double resistance, double capacitance) {
+  hlb_iolet_desc d = Base(HLB_IOLET_WK2, position, normal);
+  d.radius = radius;
+  d.area = area;
+  d.resistance = resistance;
+  d.capacitance = capacitance;
+  return d;
+}
+inline hlb_iolet_desc InOutLetWK3(const double position[3], const double normal[3], double radius, double area,
+                                  double rc, double rp, double capacitance) {
+  hlb_iolet_desc d = Base(HLB_IOLET_WK3, position, normal);
+  d.radius = radius;
+  d.area = area;
+  d.r_c = rc;
+  d.r_p = rp;
+  d.capacitance = capacitance;
+  return d;
+}
+}  // namespace iolets
+
+/* LbmParameters (src/lb/LbmParameters.h:28-45). */
+class LbmParameters {
+ public:
+  LbmParameters(PhysicalTime timeStepLength, PhysicalDistance voxelSize)
+      : tau(0.5 + (timeStepLength * 0.004 / 1000.0) / ((1.0 / 3.0) * voxelSize * voxelSize)) {}
+  explicit LbmParameters(double relaxationTime) : tau(relaxationTime) {}
+  void Update(double relaxationTime) { tau = relaxationTime; }
+  double GetTau() const { return tau; }
+  double GetOmega() const { return -1.0 / tau; }
+  double GetRelaxationParameter() const { return trtLambda; }
+  void SetRelaxationParameter(double lambda) { trtLambda = lambda; }
+
+ private:
+  double tau;
+  double trtLambda = 3.0 / 16.0;
+};
+
+/* MacroscopicPropertyCache (src/lb/MacroscopicPropertyCache.h:24-106):
+ * density, velocity (3 per site) and wall shear stress magnitude, reference
+ * site order, as of the last refreshed step. */
+struct MacroscopicPropertyCache {
+  std::vector<double> densityCache, velocityCache, wallShearStressMagnitudeCache;
+};
+
+/* LBM<Lattice> (src/lb/lb.h, lb.hpp) with the kernel and streamers of
+ * BuildSystemInterface.h given as types. */
+template <class Lattice = lattices::D3Q19, class Kernel = kernels::LBGK<Lattice>,
+          class WallStreamer = streamers::SimpleBounceBack<Kernel>,
+          class InletStreamer = streamers::NashZerothOrderPressureIolet<Kernel>,
+          class OutletStreamer = InletStreamer>
+class LBM {
+ public:
+  LBM(geometry::LatticeData& latticeData, const LbmParameters& parameters, const std::vector<hlb_iolet_desc>& inlets,
+      const std::vector<hlb_iolet_desc>& outlets, int32_t storage = HLB_FP64)
+      : latDat(latticeData), params(parameters) {
+    hlb_params p{};
+    p.kernel = Kernel::Code;
+    p.wall_bc = WallStreamer::Wall;
+    p.inlet_bc = InletStreamer::Iolet;
+    p.outlet_bc = OutletStreamer::Iolet;
+    p.precision = storage;
+    p.tau = parameters.GetTau();
+    p.trt_lambda = parameters.GetRelaxationParameter();
+    detail::Check(hlb_set_params(H(), &p));
+    detail::Check(hlb_set_iolets(H(), inlets.data(), (int)inlets.size(), outlets.data(), (int)outlets.size()));
+  }
+
+  /* EquilibriumInitialCondition / CheckpointInitialCondition::SetFs. */
+  void SetInitialConditions(LatticeDensity rho0, int64_t t0 = 1) { detail::Check(hlb_init_equilibrium(H(), rho0, t0)); }
+  void SetInitialConditions(const std::vector<distribn_t>& f, int64_t t0) {
+    detail::Check(hlb_upload_f(H(), f.data(), t0));
+  }
+
+  /* The IteratedAction phases of one step (src/lb/lb.hpp:192-347): the whole
+   * step runs in PreReceive on one GPU; the other phases have nothing left. */
+  void RequestComms() {}
+  void PreSend() {}
+  void PreReceive() { Iterate(1); }
+  void PostReceive() {}
+  void EndIteration() {}
+
+  /* n whole steps; the property cache is refreshed by the last one when
+   * requested (SimulationMaster.cc:385-400). */
+  void Iterate(int64_t n = 1) {
+    const bool props = refresh;
+    detail::Check(hlb_step(H(), n, props ? HLB_STEP_PROPERTIES : 0u));
+    if (props) {
+      const size_t N = (size_t)latDat.GetLocalFluidSiteCount();
+      cache.densityCache.resize(N);
+      cache.velocityCache.resize(3 * N);
+      cache.wallShearStressMagnitudeCache.resize(N);
+      detail::Check(hlb_get_fields(H(), cache.densityCache.data(), cache.velocityCache.data(),
+                                   cache.wallShearStressMagnitudeCache.data()));
+      refresh = false;
+    }
+  }
+  void RequestPropertyRefresh() { refresh = true; }
+  const MacroscopicPropertyCache& GetPropertyCache() const { return cache; }
+
+  /* LbmParameters::Update: tau may change between steps. */
+  void UpdateTau(double relaxationTime) {
+    hlb_params p{};
+    detail::Check(hlb_get_params(H(), &p));
+    p.tau = relaxationTime;
+    detail::Check(hlb_set_params(H(), &p));
+    params.Update(relaxationTime);
+  }
+  const LbmParameters* GetLbmParams() const { return &params; }
+  int64_t GetTimeStep() const { return hlb_timestep(H()); }
+  /* StabilityTester (src/lb/StabilityTester.h:90-148). */
+  bool IsStable() const {
+    int s = 0;
+    detail::Check(hlb_stability(H(), &s));
+    return s != 0;
+  }
+  /* The pre-collision distributions entering the next step, AoS
+   * f[site * 19 + d] (LbDataSourceIterator::GetDistribution). */
+  std::vector<distribn_t> GetDistributions() const {
+    std::vector<distribn_t> f((size_t)latDat.GetLocalFluidSiteCount() * 19);
+    detail::Check(hlb_download_f(H(), f.data()));
+    return f;
+  }
+  /* IncompressibilityChecker's tracker and the total mass (hlb_monitor). */
+  std::vector<double> Monitors() const {
+    std::vector<double> m(4);
+    detail::Check(hlb_monitor(H(), m.data()));
+    return m;
+  }
+
+ private:
+  hlb_handle H() const { return latDat.Handle(); }
+  geometry::LatticeData& latDat;
+  LbmParameters params;
+  MacroscopicPropertyCache cache;
+  bool refresh = false;
+};
+
+}  // namespace lb
+
+/* SimulationMaster (src/SimulationMaster.cc): owns the geometry, lattice and
+ * LBM of one run and advances them; property caches are refreshed every
+ * propertyPeriod steps and stability is checked every stabilityPeriod steps
+ * (an unstable state throws, where the reference aborts). */
+template <class LBMType>
+class SimulationMaster {
+ public:
+  struct Config {
+    geometry::Geometry geometry;
+    lb::LbmParameters parameters;
+    std::vector<hlb_iolet_desc> inlets, outlets;
+    LatticeDensity initialDensity = 1.0;
+    int64_t totalSteps = 0;
+    int64_t propertyPeriod = 0;
+    int64_t stabilityPeriod = 0;
+    int32_t storage = HLB_FP64;
+  };
+
+  explicit SimulationMaster(const Config& c)
+      : config(c), latticeData(c.geometry), lbm(latticeData, c.parameters, c.inlets, c.outlets, c.storage) {
+    lbm.SetInitialConditions(c.initialDensity, 1);
+  }
+
+  void DoTimeStep() {
+    const int64_t t = lbm.GetTimeStep();
+    if (config.propertyPeriod > 0 && t % config.propertyPeriod == 0) lbm.RequestPropertyRefresh();
+    lbm.RequestComms();
+    lbm.PreSend();
+    lbm.PreReceive();
+    lbm.PostReceive();
+    lbm.EndIteration();
+    if (config.stabilityPeriod > 0 && t % config.stabilityPeriod == 0 && !lbm.IsStable())
+      throw Exception("SimulationMaster: the simulation became unstable at step " + std::to_string(t));
+  }
+
+  /* Runs the configured steps.  Between property and stability steps the
+   * steps are issued in runs, so the GPU never waits on the host. */
+  void RunSimulation() {
+    while (lbm.GetTimeStep() <= config.totalSteps) {
+      const int64_t t = lbm.GetTimeStep();
+      int64_t run = config.totalSteps - t + 1;
+      for (int64_t period : {config.propertyPeriod, config.stabilityPeriod})
+        if (period > 0) run = std::min(run, period - (t % period == 0 ? period : t % period) + 1);
+      if (run > 1) {
+        lbm.Iterate(run - 1);
+      } else {
+        DoTimeStep();
+      }
+    }
+  }
+
+  geometry::LatticeData& GetLatticeData() { return latticeData; }
+  LBMType& GetLBM() { return lbm; }
+
+ private:
+  Config config;
+  geometry::LatticeData latticeData;
+  LBMType lbm;
+};
+
+}  // namespace hemelb
+
+#endif

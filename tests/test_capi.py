@@ -108,3 +108,29 @@ def test_server_validates_requests_without_device():
     rid = srv.submit(case.geom, H.params(precision=H.FP32_SHIFTED), ins, outs, max_steps=10)
     st = srv.status(rid)
     assert st.state == H.REQ_QUEUED and st.sites == case.geom.n_sites and st.device_bytes > 0
+
+
+def test_tables_first_then_configuration():
+    """LatticeData before the LBM: hlb_create without parameters builds the
+    tables; parameters and iolets follow and are validated then."""
+    g = _cyl()
+    i0, o0 = g.inlets()[0], g.outlets()[0]
+    lat = H.Lattice(g, None)
+    assert lat.n == g.n_sites and lat.class_counts().sum() == g.n_sites
+    p = lat.get_params()
+    assert (p.kernel, p.wall_bc, p.precision, p.tau) == (H.KERNEL_LBGK, H.WALL_SBB, H.FP64, 1.0)
+    with pytest.raises(H.HlbError, match="no iolets configured"):
+        lat.init_equilibrium(1.0)
+    with pytest.raises(H.HlbError, match="tau must exceed"):
+        lat.set_params(H.params(tau=0.4))
+    with pytest.raises(H.HlbError, match="names iolet"):
+        lat.set_iolets([], [H.iolet_desc(H.COSINE, o0["position"], o0["normal"])])
+    lat.set_params(H.params(kernel=H.KERNEL_TRT, inlet_bc=H.IOLET_LADD, tau=0.7))
+    with pytest.raises(H.HlbError, match="velocity iolet"):
+        lat.set_iolets([H.iolet_desc(H.COSINE, i0["position"], i0["normal"])],
+                       [H.iolet_desc(H.COSINE, o0["position"], o0["normal"])])
+    lat.set_iolets([H.iolet_desc(H.PARABOLIC, i0["position"], i0["normal"], radius=i0["radius"], max_speed=0.01)],
+                   [H.iolet_desc(H.COSINE, o0["position"], o0["normal"])])
+    # the Nash boundary takes any value provider, so switching back is allowed
+    lat.set_params(H.params(kernel=H.KERNEL_TRT, inlet_bc=H.IOLET_NASH, tau=0.75))
+    assert lat.get_params().tau == 0.75

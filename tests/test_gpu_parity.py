@@ -211,3 +211,54 @@ def test_long_table_chunks():
                 outlets=[IoletSpec(H.COSINE, o0["position"], o0["normal"])])
     lat, ref = run_both(case, 1300)
     compare_all(lat, ref, "chunks")
+
+
+def test_device_addressing_matches_host_pull_table():
+    """The slots the step kernels gather from, computed on the device from
+    the brick tables, equal the host pull table (and so the reference's push
+    table inverted) on every table geometry, whatever its block size."""
+    from test_tables import _geoms, _handle
+
+    for name, g in _geoms():
+        lat = _handle(g)
+        _, host = lat.neighbour_tables()
+        lat.init_equilibrium(1.0)
+        _, dev = lat.neighbour_tables()
+        assert np.array_equal(dev, host), f"{name}: {int((dev != host).sum())} entries differ"
+
+
+def test_reference_bifurcation_case_from_gmy():
+    """The reference's own case geometry (cases/bifurcation/bifurcation_lores,
+    read from its .gmy) with its XML iolets, LBGK + SBB + Nash, FP64 bitwise
+    against the oracle over 60 steps."""
+    import json
+    from pathlib import Path
+
+    golden = Path(__file__).resolve().parent / "golden"
+    g = H.Geometry.read_gmy(golden / "bifurcation_lores.gmy")
+    cfg = json.loads((golden / "bifurcation_lores_iolets.json").read_text())
+    spec = lambda d: IoletSpec(H.COSINE, tuple(d["position"]), tuple(d["normal"]), density_mean=d["density_mean"],
+                               density_amp=d["density_amp"], phase=d["phase"], period=d["period"])
+    case = Case(g, tau=cfg["tau"], inlets=[spec(d) for d in cfg["inlets"]],
+                outlets=[spec(d) for d in cfg["outlets"]])
+    lat, ref = run_both(case, 60)
+    compare_all(lat, ref, "bifurcation_lores.gmy")
+
+
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32_SHIFTED])
+def test_device_monitors(precision):
+    """hlb_monitor against the host fields: density range and peak speed
+    exactly (min/max/sqrt are order-free), the FP64 mass sum to rounding, and
+    the same bits on every call."""
+    import math
+
+    case = cylinder_case(radius=8.0, length=48, box_xy=24)
+    lat = case.product(precision=precision)
+    lat.init_equilibrium(1.0)
+    lat.step(200, properties=True)
+    rho, u, _ = lat.fields()
+    m = lat.monitor()
+    assert m["rho_min"] == rho.min() and m["rho_max"] == rho.max()
+    assert m["u_max"] == np.sqrt(u[:, 0] * u[:, 0] + u[:, 1] * u[:, 1] + u[:, 2] * u[:, 2]).max()
+    assert abs(m["mass"] - math.fsum(rho)) <= 1e-13 * abs(m["mass"])
+    assert lat.monitor() == m
