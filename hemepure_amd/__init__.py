@@ -18,7 +18,7 @@ LIB_PATH = PKG / "libhemepure_amd.so"
 # Constants of include/hemepure_amd.h.
 LINK_NONE, LINK_WALL, LINK_INLET, LINK_OUTLET = 0, 1, 2, 3
 KERNEL_LBGK, KERNEL_TRT, KERNEL_LES = 0, 1, 2
-WALL_SBB, WALL_BFL = 0, 1
+WALL_SBB, WALL_BFL, WALL_GZS = 0, 1, 2
 IOLET_NASH, IOLET_LADD = 0, 1
 COSINE, PARABOLIC, FILEVEL, WK2, WK3 = 0, 1, 2, 3, 4
 FP64, FP32_SHIFTED = 0, 1

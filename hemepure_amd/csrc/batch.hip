@@ -122,6 +122,11 @@ int enqueue_sequence(hlb_batch* b, int64_t n) {
   for (int64_t s = 0; s < n; ++s) {
     HLB_CHECK(launch_wk_begin(b->refs.p, b->nwk, (int)s, b->stream));
     HLB_CHECK(launch_step_fp32(b->mem.p, b->wg_member.p, b->grid, (int)s, b->variant, b->stream));
+    for (size_t i = 0; i < b->members.size(); ++i) {  // GZS walls, per member
+      const hlb_lattice* m = b->members[i];
+      if (!m->gz_links_host.empty())
+        HLB_CHECK(launch_gzs_fp32(b->mem.p + i, m->gz_links.p, (int)m->gz_links_host.size(), (int)s, b->stream));
+    }
     HLB_CHECK(launch_wk_end(b->refs.p, b->nwk, b->stream));
   }
   return HLB_OK;

@@ -77,6 +77,9 @@ StepArgs make_args(hlb_lattice* h, const double* ain, double* aout) {
     a.aout[j] = aout ? aout + (size_t)j * h->S : nullptr;
   }
   a.bt = h->tables();
+  a.gz = h->gz_tables();
+  a.P = h->P64.p;
+  a.GW = h->GW64.p;
   a.bwall = h->bwall.p;
   a.biolet = h->biolet.p;
   a.bgi = h->bgi.p;
@@ -314,6 +317,90 @@ void windkessel_setup(hlb_lattice* h) {
   }
 }
 
+// Guo-Zheng-Shi wall links (GuoZhengShiDelegate.h:123-294): which wall links
+// take a GZS value and which branch they follow (the rest bounce back), the
+// sites whose pre-collision f the link kernel reads, numbered in slot order,
+// and their per-slab masks.
+int gzs_setup(hlb_lattice* h) {
+  const HostLattice& L = h->L;
+  h->gz_links_host.clear();
+  h->h_pmask.clear();
+  h->h_pbase.clear();
+  h->h_bgz.assign((size_t)h->SB, 0);
+  h->h_bgzbase.assign((size_t)h->SB, 0);
+  h->NP = 0;
+  if (h->params.wall_bc != HLB_WALL_GZS) return HLB_OK;
+  struct Pending {
+    int64_t k, nb;
+    int d, mode, gi;
+    float q;
+    double s;
+  };
+  std::vector<Pending> pend;
+  std::vector<uint8_t> needP(L.N, 0);
+  for (int64_t b = 0; b < h->NB; ++b) {
+    const int64_t k = h->bsite[b];
+    const uint32_t wm = L.wall[k], im = L.iolet[k];
+    h->h_bgzbase[b] = (int32_t)pend.size();
+    for (int d = 1; d < Q; ++d) {
+      if (!((wm >> (d - 1)) & 1u)) continue;
+      const int i = INV[d];
+      const float q = L.q[k * 18 + d - 1];
+      int mode = 1, gi = -1;
+      int64_t nb = -1;
+      double sfac = 0.0;
+      if ((double)q < 0.75) {
+        if ((im >> (i - 1)) & 1u) {
+          gi = side_iolet(h, k);
+          const HostIolet& io = h->iolets[gi];
+          if (io.d.kind != HLB_IOLET_PARABOLIC && io.d.kind != HLB_IOLET_FILEVEL) continue;  // bounce-back
+          const double x = (double)L.coords[3 * k] + CX[i], y = (double)L.coords[3 * k + 1] + CY[i],
+                       z = (double)L.coords[3 * k + 2] + CZ[i];
+          const double dx = x - io.d.position[0], dy = y - io.d.position[1], dz = z - io.d.position[2];
+          const double zz = dx * io.n[0] + dy * io.n[1] + dz * io.n[2];
+          const double rSq = (dx * dx + dy * dy + dz * dz) - zz * zz;
+          const double r2a2 = rSq / (io.d.radius * io.d.radius);
+          if (r2a2 > 1.0) {
+            char buf[200];
+            std::snprintf(buf, sizeof buf, "An IOLET site with r = %lf lies outside the IOLET radius %lf.",
+                          std::sqrt(rSq), io.d.radius);
+            return fail(HLB_ERANGE, buf);
+          }
+          mode = 3;
+          sfac = 1. - r2a2;
+        } else if ((wm >> (i - 1)) & 1u) {
+          continue;  // bounce-back
+        } else {
+          nb = L.site_at(L.coords[3 * k] + CX[i], L.coords[3 * k + 1] + CY[i], L.coords[3 * k + 2] + CZ[i]);
+          if (nb < 0) return fail(HLB_EGEOMETRY, "hlb_create: GZS needs a fluid site beyond a wall link's opposite");
+          mode = 2;
+          needP[nb] = 1;
+        }
+      }
+      needP[k] = 1;
+      h->h_bgz[b] |= 1u << (d - 1);
+      pend.push_back({k, nb, d, mode, gi, q, sfac});
+    }
+  }
+  std::vector<int32_t> pidx(L.N, -1);
+  for (size_t r = 0; r < h->h_slab.size(); ++r) {
+    const int64_t base = (int64_t)(h->h_slab[r] >> 3) * kBrickSites + (int64_t)(h->h_slab[r] & 7) * kSlab;
+    uint64_t pm = 0;
+    h->h_pbase.push_back((int32_t)h->NP);
+    for (int lane = 0; lane < kSlab; ++lane) {
+      const int32_t k = h->perm[base + lane];
+      if (k < 0 || !needP[k]) continue;
+      pm |= 1ull << lane;
+      pidx[k] = (int32_t)h->NP++;
+    }
+    h->h_pmask.push_back(pm);
+  }
+  for (const Pending& p : pend)
+    h->gz_links_host.push_back(GzsLink{pidx[p.k], p.nb >= 0 ? pidx[p.nb] : -1, p.d, p.mode, p.gi, p.q, p.s});
+  h->SP = round_up(std::max<int64_t>(h->NP, 1), 64);
+  return HLB_OK;
+}
+
 int upload_lattice(hlb_lattice* h) {
   const HostLattice& L = h->L;
   const int64_t N = L.N, S = h->S, SB = h->SB, SI = h->SI, NB = h->NB;
@@ -352,6 +439,31 @@ int upload_lattice(hlb_lattice* h) {
   HLB_CHECK(hipMemsetAsync(h->u.p, 0, sizeof(double) * 3 * S, h->stream));
   HLB_CHECK(hipMemsetAsync(h->wss.p, 0, sizeof(double) * S, h->stream));
 
+  // Guo-Zheng-Shi tables.
+  {
+    const int rc = gzs_setup(h);
+    if (rc != HLB_OK) return rc;
+    if (!h->gz_links_host.empty()) {
+      const size_t nl = h->gz_links_host.size();
+      HLB_CHECK(h->gz_links.alloc(nl));
+      HLB_CHECK(hipMemcpy(h->gz_links.p, h->gz_links_host.data(), sizeof(GzsLink) * nl, hipMemcpyHostToDevice));
+      HLB_CHECK(h->pmask.alloc(h->h_pmask.size()));
+      HLB_CHECK(hipMemcpy(h->pmask.p, h->h_pmask.data(), sizeof(uint64_t) * h->h_pmask.size(), hipMemcpyHostToDevice));
+      HLB_CHECK(h->pbase.alloc(h->h_pbase.size()));
+      HLB_CHECK(hipMemcpy(h->pbase.p, h->h_pbase.data(), sizeof(int32_t) * h->h_pbase.size(), hipMemcpyHostToDevice));
+      HLB_CHECK(h->bgz.alloc((size_t)SB));
+      HLB_CHECK(hipMemcpy(h->bgz.p, h->h_bgz.data(), sizeof(uint32_t) * SB, hipMemcpyHostToDevice));
+      HLB_CHECK(h->bgzbase.alloc((size_t)SB));
+      HLB_CHECK(hipMemcpy(h->bgzbase.p, h->h_bgzbase.data(), sizeof(int32_t) * SB, hipMemcpyHostToDevice));
+      if (h->fp32()) {
+        HLB_CHECK(h->P32.alloc((size_t)Q * h->SP));
+        HLB_CHECK(h->GW32.alloc(nl));
+      } else {
+        HLB_CHECK(h->P64.alloc((size_t)Q * h->SP));
+        HLB_CHECK(h->GW64.alloc(nl));
+      }
+    }
+  }
   // Brick tables.
   {
     const size_t ns = std::max<size_t>(1, h->h_slab.size());
@@ -470,7 +582,10 @@ namespace {
 int validate_core(const hlb_params& p, const char* who) {
   const std::string w(who);
   if (p.kernel < HLB_KERNEL_LBGK || p.kernel > HLB_KERNEL_LES) return fail(HLB_EINVAL, w + ": unknown kernel");
-  if (p.wall_bc != HLB_WALL_SBB && p.wall_bc != HLB_WALL_BFL) return fail(HLB_EINVAL, w + ": unknown wall BC");
+  if (p.wall_bc != HLB_WALL_SBB && p.wall_bc != HLB_WALL_BFL && p.wall_bc != HLB_WALL_GZS)
+    return fail(HLB_EINVAL, w + ": unknown wall BC");
+  if (p.wall_bc == HLB_WALL_GZS && p.kernel == HLB_KERNEL_LES)
+    return fail(HLB_EINVAL, w + ": GZS walls are not defined with the LES kernel (the wall node has no LES tau)");
   for (int bc : {p.inlet_bc, p.outlet_bc})
     if (bc != HLB_IOLET_NASH && bc != HLB_IOLET_LADD) return fail(HLB_EINVAL, w + ": unknown iolet BC");
   if (p.precision != HLB_FP64 && p.precision != HLB_FP32_SHIFTED)
@@ -614,6 +729,9 @@ Member32 member_record(const hlb_lattice* h, int props_at) {
   m.A[1] = h->A32[1].p;
   m.F = h->F32.p;
   m.bt = h->tables();
+  m.gz = h->gz_tables();
+  m.P = h->P32.p;
+  m.GW = h->GW32.p;
   m.bwall = h->bwall.p;
   m.biolet = h->biolet.p;
   m.bgi = h->bgi.p;
@@ -698,6 +816,7 @@ int enqueue_steps(hlb_lattice* h, int64_t n, bool props_last) {
     for (int64_t s = 0; s < n; ++s) {
       HLB_CHECK(launch_wk_begin(h->wk_refs.p, nwk, (int)s, h->stream));
       HLB_CHECK(launch_step_fp32(h->mem.p, nullptr, grid, (int)s, kFp32DefaultVariant, h->stream));
+      HLB_CHECK(launch_gzs_fp32(h->mem.p, h->gz_links.p, (int)h->gz_links_host.size(), (int)s, h->stream));
       HLB_CHECK(launch_wk_end(h->wk_refs.p, nwk, h->stream));
     }
     advance_state(h, n);
@@ -713,6 +832,8 @@ int enqueue_steps(hlb_lattice* h, int64_t n, bool props_last) {
     a.io_vmax = h->io_vmax.p + s * nio;
     HLB_CHECK(launch_wk_begin(h->wk_refs.p, nwk, (int)s, h->stream));
     HLB_CHECK(launch_step_fp64(a, h->params.kernel, first, props, h->stream));
+    HLB_CHECK(launch_gzs_fp64(a, h->params.kernel, h->gz_links.p, (int)h->gz_links_host.size(), a.io_vmax,
+                              h->GW64.p, h->stream));
     HLB_CHECK(launch_wk_end(h->wk_refs.p, nwk, h->stream));
     if (!first) h->cur ^= 1;
     h->have_post = true;

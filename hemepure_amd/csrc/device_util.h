@@ -59,6 +59,7 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
 
 // One lane of a slab: its slot, block-local coordinates and boundary index.
 struct Lane {
+  uint32_t r;        // slab record (wave-uniform)
   uint32_t slot;
   int x, y, z;       // x is wave-uniform
   bool boundary;
@@ -76,6 +77,7 @@ __device__ __forceinline__ bool slab_lane(const BrickTables& bt, uint32_t r, Lan
   if (!((fm >> lane) & 1u)) return false;
   const uint64_t bm = uniform64(ldp(bt.bmask, r));
   const uint32_t brick = (uint32_t)sid >> 3;
+  L.r = r;
   L.x = sid & 7;
   L.y = (int)(lane >> 3);
   L.z = (int)(lane & 7u);
@@ -88,6 +90,12 @@ __device__ __forceinline__ bool slab_lane(const BrickTables& bt, uint32_t r, Lan
   }
   L.row = bt.nbr + (int64_t)brick * 27;
   return true;
+}
+
+// Index among the lanes of the slab set in mask (uniform) below this lane.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  const uint32_t lane = threadIdx.x & 63u;
+  return (uint32_t)__builtin_popcountll(mask & ((1ull << lane) - 1ull));
 }
 
 // Slot of the upstream site x - c_J of a lane (pull direction J).  The brick

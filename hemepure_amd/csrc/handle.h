@@ -16,6 +16,9 @@
 namespace hlb {
 
 hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, hipStream_t s);
+hipError_t launch_gzs_fp64(const StepArgs& a, int kernel, const GzsLink* links, int n, const double* vmax,
+                           double* GW, hipStream_t s);
+hipError_t launch_gzs_fp32(const Member32* members, const GzsLink* links, int n, int step, hipStream_t s);
 hipError_t launch_upstream_table(const BrickTables& bt, const uint32_t* bwall, const uint32_t* biolet, uint32_t* out,
                                  int64_t S, hipStream_t s);
 size_t monitor_scratch(int64_t nslab);
@@ -110,6 +113,17 @@ struct hlb_lattice {
   hlb::DevBuf<uint64_t> fmask, bmask;
   hlb::DevBuf<float> bq, bnrm, io_nf;
   hlb::DevBuf<int> flag;
+  // Guo-Zheng-Shi walls (params.wall_bc == HLB_WALL_GZS).
+  std::vector<hlb::GzsLink> gz_links_host;
+  std::vector<uint64_t> h_pmask;
+  std::vector<int32_t> h_pbase, h_bgz, h_bgzbase;
+  int64_t NP = 0, SP = 0;
+  hlb::DevBuf<hlb::GzsLink> gz_links;
+  hlb::DevBuf<uint64_t> pmask;
+  hlb::DevBuf<int32_t> pbase, bgzbase;
+  hlb::DevBuf<uint32_t> bgz;
+  hlb::DevBuf<double> P64, GW64;
+  hlb::DevBuf<float> P32, GW32;
   hlb::DevBuf<char> mon_scratch;  // monitor partials, allocated on first use
   hlb::DevBuf<double> mon_out;
   // FP32-shifted storage (params.precision == HLB_FP32_SHIFTED): float A, F,
@@ -150,6 +164,10 @@ struct hlb_lattice {
     if (h_mem) (void)hipHostFree(h_mem);
   }
   bool fp32() const { return params.precision == HLB_FP32_SHIFTED; }
+  hlb::GzsTables gz_tables() const {
+    if (gz_links_host.empty()) return hlb::GzsTables{nullptr, nullptr, nullptr, nullptr, 0};
+    return hlb::GzsTables{pmask.p, pbase.p, bgz.p, bgzbase.p, SP};
+  }
   hlb::BrickTables tables() const {
     return hlb::BrickTables{nbr.p, slab.p, fmask.p, bmask.p, bbase.p, nslab};
   }

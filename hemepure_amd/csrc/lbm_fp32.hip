@@ -146,12 +146,22 @@ __device__ __forceinline__ void boundary_fixup32(const Member32& m, const float*
     for (int j = 1; j < Q; ++j)
       if (im & (1u << (INV[j] - 1))) g[j] = ldp(plane(m.G, m.SI, j), i);
   }
+  uint32_t gzm = 0;
+  int32_t gzb = 0;
+  if (m.gz.bgz != nullptr && wm != 0) {
+    gzm = ldp(m.gz.bgz, b);
+    gzb = ldp(m.gz.bgzbase, b);
+  }
   if (wm != 0) {
 #pragma unroll
     for (int j = 1; j < Q; ++j) {
       const int ib = INV[j];
       const uint32_t bit = 1u << (ib - 1), bitJ = 1u << (j - 1);
       if ((wm & bit) && !(im & bit)) {
+        if (gzm & bit) {
+          g[j] = ldp(m.GW, (uint32_t)(gzb + __builtin_popcount(gzm & (bit - 1u))));
+          continue;
+        }
         const float qd = bfl ? q[ib] : 0.0f;
         if (!bfl || (wm & bitJ) || qd < 0.5f) g[j] = own[ib];
         else g[j] = (own[ib] + (2.0f * qd - 1.0f) * own[j]) / (2.0f * qd);
@@ -255,6 +265,14 @@ __global__ __launch_bounds__(B) void step_fp32(const Member32* __restrict__ memb
     gather_up32(P.in, m.S, L, wm | im, g);
     if (isB) boundary_fixup32(m, P.in, d, L.b, wm, im, g);
   }
+  if (m.gz.pmask != nullptr) {
+    const uint64_t pm = uniform64(ldp(m.gz.pmask, L.r));
+    if ((pm >> (threadIdx.x & 63u)) & 1u) {
+      const uint32_t pi = (uint32_t)__builtin_amdgcn_readfirstlane(ldp(m.gz.pbase, L.r)) + lanes_below(pm);
+#pragma unroll
+      for (int j = 0; j < Q; ++j) stp(plane(m.P, m.gz.SP, j), pi, g[j]);
+    }
+  }
 
   float drho, jx, jy, jz;
   moments32(g, drho, jx, jy, jz);
@@ -320,6 +338,66 @@ __global__ __launch_bounds__(B) void step_fp32(const Member32* __restrict__ memb
     stp(plane(m.u, m.S, 2), d, (double)uz);
     stp(m.wss, d, w == FLT_MAX ? DBL_MAX : (double)w);
   }
+}
+
+// Guo-Zheng-Shi links in the shifted form (see gzs_links_fp64): the wall
+// node keeps this site's density, so its shifted equilibrium uses the same
+// drho.
+__global__ __launch_bounds__(kBlock) void gzs_links_fp32(const Member32* __restrict__ members,
+                                                         const GzsLink* __restrict__ links, int n, int step) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= n) return;
+  const Member32& m = members[0];
+  const GzsLink lk = links[e];
+  const int d = lk.dir, i = INV[d];
+  const float q = lk.q;
+  float g[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) g[j] = ldp(plane(m.P, m.gz.SP, j), (uint32_t)lk.site_p);
+  float drho, jx, jy, jz;
+  moments32(g, drho, jx, jy, jz);
+  const float rho = 1.0f + drho, inv = 1.0f / rho;
+  float geq[Q], gneq[Q];
+  geq_all(drho, inv, 1.5f * (jx * jx + jy * jy + jz * jz) * inv, jx, jy, jz, geq);
+#pragma unroll
+  for (int j = 0; j < Q; ++j) gneq[j] = g[j] - geq[j];
+  const float fac = 1.0f - 1.0f / q;
+  float wj[3] = {jx * fac, jy * fac, jz * fac};
+  if (lk.mode == 2) {
+    float h[Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) h[j] = ldp(plane(m.P, m.gz.SP, j), (uint32_t)lk.nb_p);
+    float hd, hx, hy, hz;
+    moments32(h, hd, hx, hy, hz);
+    const float hr = 1.0f + hd, hinv = 1.0f / hr;
+    float heq[Q];
+    geq_all(hd, hinv, 1.5f * (hx * hx + hy * hy + hz * hz) * hinv, hx, hy, hz, heq);
+    const float hv[3] = {hx * hinv, hy * hinv, hz * hinv};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) wj[c] = q * wj[c] + (1.0f - q) * rho * (hv[c] * (q - 1.0f) / (q + 1.0f));
+#pragma unroll
+    for (int j = 0; j < Q; ++j) gneq[j] = q * gneq[j] + (1.0f - q) * (h[j] - heq[j]);
+  } else if (lk.mode == 3) {
+    const double* vmax = m.io_vmax + (int64_t)step * m.nio;
+    const float sp = (float)(ldp(vmax, (uint32_t)lk.gi) * lk.s);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float v = (float)ldp(m.io_n, 3u * lk.gi + c) * sp;
+      wj[c] = q * wj[c] + (1.0f - q) * rho * (v * (q - 1.0f) / (q + 1.0f));
+    }
+  }
+  float weq[Q];
+  geq_all(drho, inv, 1.5f * (wj[0] * wj[0] + wj[1] * wj[1] + wj[2] * wj[2]) * inv, wj[0], wj[1], wj[2], weq);
+  float val;
+  if (m.kernel == HLB_KERNEL_TRT) {
+    const int lo = i < INV[i] ? i : INV[i], hi = i < INV[i] ? INV[i] : i;
+    const float sym = m.half_omega * (gneq[lo] + gneq[hi]);
+    const float asym = m.half_omega_minus * (gneq[lo] - gneq[hi]);
+    val = i == lo ? weq[lo] + gneq[lo] + sym + asym : weq[hi] + gneq[hi] + sym - asym;
+  } else {
+    val = weq[i] + gneq[i] + gneq[i] * m.omega;
+  }
+  stp(const_cast<float*>(m.GW), (uint32_t)e, val);
 }
 
 // Pre-collision g entering the next step into F (device order), for
@@ -424,6 +502,12 @@ hipError_t launch_step_fp32(const Member32* members, const int32_t* wg_member, u
 static uint32_t slab_grid4(int64_t nslab) {
   const int64_t wg = (nslab + kBlock / kSlab - 1) / (kBlock / kSlab);
   return (uint32_t)((wg + kXcds - 1) / kXcds * kXcds);
+}
+
+hipError_t launch_gzs_fp32(const Member32* members, const GzsLink* links, int n, int step, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gzs_links_fp32, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, members, links, n, step);
+  return hipGetLastError();
 }
 
 hipError_t launch_gather_fp32(const Member32* members, int64_t nslab, float* F, hipStream_t s) {

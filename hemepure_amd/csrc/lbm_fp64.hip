@@ -72,12 +72,23 @@ __device__ __forceinline__ void boundary_fixup(const StepArgs& a, uint32_t d, ui
     for (int j = 1; j < Q; ++j)
       if (im & (1u << (INV[j] - 1))) f[j] = ldp(plane(a.G, a.SI, j), i);
   }
+  uint32_t gzm = 0;
+  int32_t gzb = 0;
+  if (a.gz.bgz != nullptr && wm != 0) {
+    gzm = ldp(a.gz.bgz, b);
+    gzb = ldp(a.gz.bgzbase, b);
+  }
   if (wm != 0) {
 #pragma unroll
     for (int j = 1; j < Q; ++j) {
       const int ib = INV[j];
       const uint32_t bit = 1u << (ib - 1), bitJ = 1u << (j - 1);
       if ((wm & bit) && !(im & bit)) {
+        if (gzm & bit) {
+          // GuoZhengShiDelegate: the link kernel's value.
+          f[j] = ldp(a.GW, (uint32_t)(gzb + __builtin_popcount(gzm & (bit - 1u))));
+          continue;
+        }
         // SimpleBounceBackDelegate.h:36-44; BFL StreamLink, :44-70.
         const double qd = bfl ? (double)q[ib] : 0.0;
         if (!bfl || (wm & bitJ) || qd < 0.5) f[j] = own[ib];
@@ -188,6 +199,15 @@ __global__ __launch_bounds__(kBlock) void step_fp64(StepArgs a) {
     gather_site(a, L, wm, im, f);
     if (isB) boundary_fixup(a, d, L.b, wm, im, f);
   }
+  if (a.gz.pmask != nullptr) {
+    // Sites the Guo-Zheng-Shi link kernel reads keep their pre-collision f.
+    const uint64_t pm = uniform64(ldp(a.gz.pmask, L.r));
+    if ((pm >> (threadIdx.x & 63u)) & 1u) {
+      const uint32_t pi = (uint32_t)__builtin_amdgcn_readfirstlane(ldp(a.gz.pbase, L.r)) + lanes_below(pm);
+#pragma unroll
+      for (int j = 0; j < Q; ++j) stp(plane(a.P, a.gz.SP, j), pi, f[j]);
+    }
+  }
 
   // Moments and equilibrium (Lattice.h:933-948), collision.
   double rho, mx, my, mz;
@@ -281,6 +301,73 @@ __global__ __launch_bounds__(kBlock) void gather_fp64(StepArgs a, double* __rest
   if (L.boundary) boundary_fixup(a, L.slot, L.b, wm, im, f);
 #pragma unroll
   for (int j = 0; j < Q; ++j) stp(plane(F, S, j), L.slot, f[j]);
+}
+
+// GuoZhengShiDelegate::StreamLink (GuoZhengShiDelegate.h:123-294), one
+// thread per GZS wall link, from the pre-collision f the step stored in P:
+// the wall node's momentum is this site's scaled by 1 - 1/q, interpolated
+// (q < 0.75) with the next fluid site out or a velocity iolet's profile; its
+// f_neq likewise (next site only); the node is collided with the run's
+// kernel and its value in direction i = INV[d] is the site's next input.
+template <int KERNEL>
+__global__ __launch_bounds__(kBlock) void gzs_links_fp64(StepArgs a, const GzsLink* __restrict__ links, int n,
+                                                         const double* __restrict__ vmax, double* __restrict__ GW) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= n) return;
+  const GzsLink lk = links[e];
+  const int d = lk.dir, i = INV[d];
+  const double q = (double)lk.q;
+  double f[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) f[j] = ldp(plane(a.P, a.gz.SP, j), (uint32_t)lk.site_p);
+  double rho, mx, my, mz;
+  moments(f, rho, mx, my, mz);
+  double feq[Q], fneq[Q];
+  feq_all(feq_pre(rho, mx, my, mz), mx, my, mz, feq);
+#pragma unroll
+  for (int j = 0; j < Q; ++j) fneq[j] = f[j] - feq[j];
+  const double fac = 1. - 1. / q;
+  double wm[3] = {mx * fac, my * fac, mz * fac};
+  if (lk.mode == 2) {
+    double g[Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) g[j] = ldp(plane(a.P, a.gz.SP, j), (uint32_t)lk.nb_p);
+    double nr, nx, ny, nz;
+    moments(g, nr, nx, ny, nz);
+    const double nv[3] = {nx / nr, ny / nr, nz / nr};
+    double gfeq[Q];
+    feq_all(feq_pre(nr, nx, ny, nz), nx, ny, nz, gfeq);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double second = nv[c] * (q - 1) / (q + 1);
+      wm[c] = q * wm[c] + (1. - q) * rho * second;
+    }
+#pragma unroll
+    for (int j = 0; j < Q; ++j) fneq[j] = q * fneq[j] + (1. - q) * (g[j] - gfeq[j]);
+  } else if (lk.mode == 3) {
+    const double sp = ldp(vmax, (uint32_t)lk.gi) * lk.s;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double v = ldp(a.io_n, 3u * lk.gi + c) * sp;
+      const double second = v * (q - 1) / (q + 1);
+      wm[c] = q * wm[c] + (1. - q) * rho * second;
+    }
+  }
+  double weq[Q];
+  feq_all(feq_pre(rho, wm[0], wm[1], wm[2]), wm[0], wm[1], wm[2], weq);
+  double fw[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) fw[j] = weq[j] + fneq[j];
+  double val;
+  if constexpr (KERNEL == HLB_KERNEL_TRT) {
+    const int lo = i < INV[i] ? i : INV[i], hi = i < INV[i] ? INV[i] : i;
+    const double sym = a.half_omega * (fneq[lo] + fneq[hi]);
+    const double asym = a.half_omega_minus * (fneq[lo] - fneq[hi]);
+    val = i == lo ? fw[lo] + sym + asym : fw[hi] + sym - asym;
+  } else {
+    val = fw[i] + fneq[i] * a.omega;
+  }
+  stp(GW, (uint32_t)e, val);
 }
 
 // The slot each lane gathers direction J from (the lane's own slot for a
@@ -492,6 +579,15 @@ hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool prop
   }
 #undef HLB_KSEL
 #undef HLB_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_gzs_fp64(const StepArgs& a, int kernel, const GzsLink* links, int n, const double* vmax,
+                           double* GW, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
+  if (kernel == HLB_KERNEL_TRT) hipLaunchKernelGGL(gzs_links_fp64<HLB_KERNEL_TRT>, grid, block, 0, s, a, links, n, vmax, GW);
+  else hipLaunchKernelGGL(gzs_links_fp64<HLB_KERNEL_LBGK>, grid, block, 0, s, a, links, n, vmax, GW);
   return hipGetLastError();
 }
 

@@ -60,6 +60,28 @@ int fp32_variant_block(int variant);
 int fp32_variant_slabs(int variant);  // slabs per workgroup tile
 bool fp32_variant_remap(int variant);
 
+// Guo-Zheng-Shi wall links (GuoZhengShiDelegate.h:123-294).  Sites whose
+// pre-collision f the link kernel needs (GZS wall sites and the next fluid
+// site out along a link) store it in P during the step; the link kernel
+// then forms each GZS link's value for the next step in GW.
+struct GzsTables {
+  const uint64_t* pmask;  // per active slab: lanes that store their pre-collision f
+  const int32_t* pbase;   // per active slab: P index of the first such lane
+  const uint32_t* bgz;    // per boundary site: wall links whose value comes from GW
+  const int32_t* bgzbase; // per boundary site: GW index of its first GZS link
+  int64_t SP;             // P plane stride
+};
+
+struct GzsLink {
+  int32_t site_p;   // P index of the wall site
+  int32_t nb_p;     // P index of the next site out (regular branch), else -1
+  int32_t dir;      // wall link direction d (the value goes to INV[d])
+  int32_t mode;     // 1: this site only, 2: with the next site out, 3: velocity iolet beyond
+  int32_t gi;       // global iolet index (mode 3)
+  float q;          // wall distance as stored
+  double s;         // 1 - r^2/a^2 of the iolet profile at x + c_INV[d] (mode 3)
+};
+
 struct StepArgs {
   // Per-plane base pointers, passed as separate scalars so each access is one
   // SGPR base plus a 32-bit VGPR offset.
@@ -93,6 +115,9 @@ struct StepArgs {
   double half_omega;  // 0.5*omega (TRT)
   double half_omega_minus;
   double les_k;       // 18 sqrt(2) C_s^2 evaluated in the reference's order
+  GzsTables gz;       // pmask == nullptr: no GZS walls
+  double* P;
+  const double* GW;
 };
 
 // One lattice of the FP32-shifted form.  The storage holds g = f - w rho0
@@ -130,6 +155,9 @@ struct Member32 {
   uint32_t wg_begin;      // first workgroup tile of this member in the launch (tiles of whole slabs)
   float tau, omega, half_omega, half_omega_minus, les_k;
   int32_t pad_;
+  GzsTables gz;           // pmask == nullptr: no GZS walls
+  float* P;
+  const float* GW;
 };
 
 // Windkessel state of one iolet (InOutLetWK2.cc:13-18, InOutLetWK3.cc:13-18)

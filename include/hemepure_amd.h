@@ -44,6 +44,7 @@ extern "C" {
 /* HEMELB_WALL_BOUNDARY (src/lb/BuildSystemInterface.h:165-197). */
 #define HLB_WALL_SBB 0
 #define HLB_WALL_BFL 1
+#define HLB_WALL_GZS 2 /* GuoZhengShiDelegate (LBGK and TRT) */
 
 /* HEMELB_{INLET,OUTLET}_BOUNDARY and their wall-corner variants
  * (src/lb/BuildSystemInterface.h:215-331): Nash pressure or Ladd velocity. */

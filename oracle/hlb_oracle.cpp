@@ -559,6 +559,12 @@ struct orc_lbm {
 
 extern "C" orc_lbm* orc_lbm_create(orc_lattice* l, const orc_params* p, const orc_iolet* inl, int n_in,
                                    const orc_iolet* outl, int n_out, char* err, int errlen) {
+  if (p->wall_bc == ORC_WALL_GZS && p->kernel == ORC_KERNEL_LES) {
+    // The wall node's HydroVars carry no LES relaxation time
+    // (GuoZhengShiDelegate.h:150-153 builds them from density and momentum).
+    set_err(err, errlen, "GZS walls are not defined with the LES kernel");
+    return nullptr;
+  }
   auto* m = new orc_lbm();
   m->L = l;
   m->p = *p;
@@ -638,6 +644,75 @@ extern "C" void orc_set_f(orc_lbm* m, const double* f, int64_t t0) {
 
 namespace {
 
+// GuoZhengShiDelegate::StreamLink (src/lb/streamers/GuoZhengShiDelegate.h:
+// 123-294) for the wall link d of site k: the value the site hands to the
+// next step in direction i = INV[d] is the post-collision state of a wall
+// node extrapolated from this site (and, when the wall is closer than 0.75,
+// from the next fluid site out along i, or from a velocity iolet's profile
+// there); it falls back to simple bounce-back where a wall or pressure iolet
+// blocks that second site.
+bool GzsLink(orc_lbm* m, int64_t k, const SiteData& s, const Hydro& h, int d, const std::vector<Iolet>& iolets,
+             std::string& msg) {
+  const orc_lattice* L = m->L;
+  const bool sse3 = m->p.sse3_order != 0;
+  double* fNew = m->fNew.data();
+  const int i = INV[d];
+  const double q = L->dist[k * 18 + d - 1];
+  Hydro w;
+  w.rho = h.rho;
+  const double fac = 1. - 1. / q;
+  w.mx = h.mx * fac;
+  w.my = h.my * fac;
+  w.mz = h.mz * fac;
+  for (int j = 0; j < Q; ++j) w.fneq[j] = h.fneq[j];
+  if (q < 0.75) {
+    if (s.HasIolet(i)) {
+      const Iolet& io = iolets[s.ioletId];
+      if (!io.IsVelocity()) {
+        fNew[k * Q + i] = h.fpost[d];  // SimpleBounceBackDelegate::StreamLink
+        return true;
+      }
+      // Modified GZS: the iolet's imposed velocity at the next site out.
+      double pos[3] = {(double)L->coords[3 * k] + CX[i], (double)L->coords[3 * k + 1] + CY[i],
+                       (double)L->coords[3 * k + 2] + CZ[i]};
+      double v[3];
+      if (!IoletVelocity(io, pos, m->timeStep, v, msg)) return false;
+      double* wm[3] = {&w.mx, &w.my, &w.mz};
+      for (int a = 0; a < 3; ++a) {
+        const double second = v[a] * (q - 1) / (q + 1);
+        *wm[a] = q * *wm[a] + (1. - q) * h.rho * second;
+      }
+    } else if (s.HasWall(i)) {
+      fNew[k * Q + i] = h.fpost[d];  // SimpleBounceBackDelegate::StreamLink
+      return true;
+    } else {
+      const int64_t t = L->nbr[k * Q + i];
+      if (t >= L->N * Q) {
+        msg = "GZS: the next site out of a wall link is solid without a wall link";
+        return false;
+      }
+      const double* nf = &m->fOld[(t / Q) * Q];
+      double nrho, nmx, nmy, nmz, nfeq[Q];
+      Moments(sse3, nf, nrho, nmx, nmy, nmz);
+      const double nv[3] = {nmx / nrho, nmy / nrho, nmz / nrho};
+      Feq(sse3, nrho, nmx, nmy, nmz, nfeq);
+      double* wm[3] = {&w.mx, &w.my, &w.mz};
+      for (int a = 0; a < 3; ++a) {
+        const double second = nv[a] * (q - 1) / (q + 1);
+        *wm[a] = q * *wm[a] + (1. - q) * h.rho * second;
+      }
+      for (int j = 0; j < Q; ++j) w.fneq[j] = q * w.fneq[j] + (1. - q) * (nf[j] - nfeq[j]);
+    }
+  }
+  Feq(sse3, w.rho, w.mx, w.my, w.mz, w.feq);
+  double fw[Q];
+  for (int j = 0; j < Q; ++j) fw[j] = w.feq[j] + w.fneq[j];
+  w.tau = h.tau;
+  Collide(m->p.kernel, m->p.tau, m->omega, m->p.trt_lambda, fw, w);
+  fNew[k * Q + i] = w.fpost[i];
+  return true;
+}
+
 // One site of StreamAndCollide for its class, following the three factories
 // of src/lb/streamers/StreamerTypeFactory.h:48-83 (wall), 134-169 (iolet),
 // 232-276 (wall + iolet) and SimpleCollideAndStream.h:39-68 (bulk).
@@ -690,7 +765,9 @@ bool StreamSite(orc_lbm* m, int cls, int64_t k, bool props, std::string& msg) {
         fNew[k * Q + inv] = h.fpost[d] - correction;
       }
     } else if (useWall) {
-      if (m->p.wall_bc == ORC_WALL_SBB) {
+      if (m->p.wall_bc == ORC_WALL_GZS) {
+        if (!GzsLink(m, k, s, h, d, iolets, msg)) return false;
+      } else if (m->p.wall_bc == ORC_WALL_SBB) {
         // SimpleBounceBackDelegate::StreamLink, :36-44.
         fNew[k * Q + inv] = h.fpost[d];
       } else {

@@ -36,6 +36,7 @@ extern "C" {
 /* Wall BCs (HEMELB_WALL_BOUNDARY). */
 #define ORC_WALL_SBB 0
 #define ORC_WALL_BFL 1
+#define ORC_WALL_GZS 2
 
 /* Iolet BCs (HEMELB_INLET_BOUNDARY / HEMELB_OUTLET_BOUNDARY). */
 #define ORC_IOLET_NASH 0

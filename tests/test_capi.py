@@ -134,3 +134,10 @@ def test_tables_first_then_configuration():
     # the Nash boundary takes any value provider, so switching back is allowed
     lat.set_params(H.params(kernel=H.KERNEL_TRT, inlet_bc=H.IOLET_NASH, tau=0.75))
     assert lat.get_params().tau == 0.75
+
+
+def test_gzs_with_les_is_rejected():
+    g = _cyl()
+    with pytest.raises(H.HlbError, match="LES"):
+        H.Lattice(g, H.params(kernel=H.KERNEL_LES, wall_bc=H.WALL_GZS), [H.iolet_desc(H.COSINE, (0, 0, 0), (0, 0, 1))],
+                  [H.iolet_desc(H.COSINE, (0, 0, 0), (0, 0, -1))])

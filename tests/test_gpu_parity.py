@@ -262,3 +262,51 @@ def test_device_monitors(precision):
     assert m["u_max"] == np.sqrt(u[:, 0] * u[:, 0] + u[:, 1] * u[:, 1] + u[:, 2] * u[:, 2]).max()
     assert abs(m["mass"] - math.fsum(rho)) <= 1e-13 * abs(m["mass"])
     assert lat.monitor() == m
+
+
+@pytest.mark.parametrize("kernel", [H.KERNEL_LBGK, H.KERNEL_TRT])
+@pytest.mark.parametrize("q", [0.4, 0.9])
+def test_guo_zheng_shi_four_cube(kernel, q):
+    """GZS walls: every branch (next-site extrapolation, single-site, bounce-back
+    fallback) on the FourCube corners, several steps, bitwise."""
+    g = four_cube(8, seed=3)
+    dist = np.where(g.link_type == H.LINK_WALL, np.float32(q), g.link_dist).astype(np.float32)
+    g = H.Geometry(g.blocks, g.coords, g.link_type, g.link_iolet, dist, g.normal_avail, g.normal, [],
+                   g.block_size)
+    case = Case(g, kernel=kernel, wall_bc=H.WALL_GZS, tau=0.7,
+                inlets=[IoletSpec(H.COSINE, (3.5, 3.5, 0.5), (0, 0, 1), density_mean=1.01, density_amp=0.002,
+                                  period=7.0)],
+                outlets=[IoletSpec(H.COSINE, (3.5, 3.5, 6.5), (0, 0, -1), density_mean=0.99)])
+    f0 = 0.05 + 0.01 * anisotropic_f(g.n_sites)
+    for steps in (1, 2, 9):
+        lat, ref = run_both(case, steps, f0=f0, t0=3)
+        compare_all(lat, ref, f"gzs k{kernel} q{q} s{steps}")
+
+
+def test_guo_zheng_shi_velocity_iolet_and_cylinder():
+    from test_oracle import _wall_iolet_layer
+
+    g = _wall_iolet_layer(q=0.4)
+    case = Case(g, wall_bc=H.WALL_GZS, inlet_bc=H.IOLET_LADD, tau=0.8,
+                inlets=[IoletSpec(H.PARABOLIC, (2.5, 2.5, 1.5), (0, 0, -1), radius=10.0, max_speed=0.02,
+                                  warmup=5)])
+    f0 = 0.05 + 0.01 * anisotropic_f(g.n_sites)
+    lat, ref = run_both(case, 12, f0=f0)
+    compare_all(lat, ref, "gzs modified")
+    cyl = cylinder_case(radius=8.0, length=40, box_xy=24, kernel=H.KERNEL_TRT, wall_bc=H.WALL_GZS)
+    lat, ref = run_both(cyl, 300)
+    compare_all(lat, ref, "gzs cylinder")
+
+
+def test_guo_zheng_shi_fp32():
+    from test_gpu_fp32 import field_errors
+
+    cyl = cylinder_case(radius=8.0, length=40, box_xy=24, kernel=H.KERNEL_TRT, wall_bc=H.WALL_GZS)
+    lat = cyl.product(precision=H.FP32_SHIFTED)
+    ref = cyl.oracle()
+    lat.init_equilibrium(1.0)
+    ref.init_equilibrium(1.0)
+    lat.step(300, properties=True)
+    ref.run(300, properties=True, threads=0)
+    e = field_errors(lat, ref, "gzs cylinder fp32")
+    assert e["u"] <= 1e-4 and e["wss"] <= 1e-4 and e["rho"] <= 1e-4

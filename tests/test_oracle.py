@@ -306,3 +306,132 @@ def test_ladd_correction_one_link():
     rho = f0[k].sum()
     corr = 2.0 * W[d] * rho * (u_wall @ C[d]) / (1.0 / 3.0)
     assert abs(m.f()[k, INV[d]] - (post[d] - corr)) < 1e-12
+
+
+@pytest.mark.parametrize("q", [0.4, 0.9])
+def test_guo_zheng_shi_expectations(q):
+    """StreamerTests::TestGuoZhengShi (src/unittests/lbtests/StreamerTests.h:434-600),
+    restated for D3Q19 on the FourCube with every wall at distance q: for a
+    wall link d the value streamed in i = INV[d] is f_eq(rho, rho u_w)[i] +
+    (1 + omega) f_neq,w, with u_w = u (1 - 1/q) and f_neq,w = f_neq[i] from
+    this site alone (q >= 0.75), or interpolated with the next fluid site out
+    along i (q < 0.75); bounce-back when a wall or pressure iolet blocks it."""
+    g = four_cube(8, seed=2)
+    dist = np.where(g.link_type == H.LINK_WALL, np.float32(q), g.link_dist).astype(np.float32)
+    g = H.Geometry(g.blocks, g.coords, g.link_type, g.link_iolet, dist, g.normal_avail, g.normal, [],
+                   g.block_size)
+    tau = 0.8
+    omega = -1.0 / tau
+    case = Case(g, wall_bc=H.WALL_GZS, tau=tau,
+                inlets=[IoletSpec(H.COSINE, (3.5, 3.5, 0.5), (0, 0, 1), density_mean=1.01)],
+                outlets=[IoletSpec(H.COSINE, (3.5, 3.5, 6.5), (0, 0, -1), density_mean=0.99)])
+    m = case.oracle()
+    f0 = 0.05 + 0.01 * anisotropic_f(m.n)
+    m.set_f(f0)
+    m.run(1)
+    got = m.f()
+    lat = m.lat
+    push = lat.push_table()
+    wall, iol, _, _ = lat.site_masks()
+    rho = f0.sum(axis=1)
+    mom = f0 @ C
+    fneq = np.array([f0[k] - helper_feq(rho[k], mom[k]) for k in range(m.n)])
+    post = np.array([helper_lbgk(f0[k], tau) for k in range(m.n)])
+    checked = {"gzs2": 0, "gzs1": 0, "sbb": 0}
+    for k in range(m.n):
+        if iol[k]:
+            continue  # wall + iolet sites: the iolet links follow the Nash delegate
+        for d in range(1, 19):
+            if not (wall[k] >> (d - 1)) & 1:
+                continue
+            i = INV[d]
+            blocked = bool((wall[k] >> (i - 1)) & 1)
+            if q < 0.75 and blocked:
+                assert abs(got[k, i] - post[k, d]) < 1e-10
+                checked["sbb"] += 1
+                continue
+            u1 = mom[k] * (1.0 - 1.0 / q) / rho[k]
+            fneq_w = fneq[k, i]
+            if q < 0.75:
+                a = push[k, i] // 19
+                u2 = mom[a] * (q - 1.0) / ((1.0 + q) * rho[a])
+                u_w = u1 * q + u2 * (1.0 - q)
+                fneq_w = q * fneq_w + (1.0 - q) * fneq[a, i]
+                checked["gzs2"] += 1
+            else:
+                u_w = u1
+                checked["gzs1"] += 1
+            want = helper_feq(rho[k], u_w * rho[k])[i] + (1.0 + omega) * fneq_w
+            assert abs(got[k, i] - want) < 1e-10, (k, d)
+    assert (checked["gzs2"] > 50 and checked["sbb"] > 10) if q < 0.75 else checked["gzs1"] > 100
+
+
+def test_gzs_rejects_les():
+    g = four_cube(6)
+    case = Case(g, kernel=H.KERNEL_LES, wall_bc=H.WALL_GZS,
+                inlets=[IoletSpec(H.COSINE, (2.5, 2.5, 0.5), (0, 0, 1))],
+                outlets=[IoletSpec(H.COSINE, (2.5, 2.5, 4.5), (0, 0, -1))])
+    with pytest.raises(RuntimeError, match="LES"):
+        case.oracle()
+
+
+def _wall_iolet_layer(nx=4, ny=4, q=0.4):
+    """One layer of sites (z = 1) between a wall below (z = 0) and a velocity
+    inlet above (z = 2); x / y faces are walls.  Every wall link d whose
+    opposite INV[d] crosses the inlet takes the modified GZS branch."""
+    coords = np.array([(x, y, 1) for x in range(1, nx + 1) for y in range(1, ny + 1)])
+    n = len(coords)
+    lt = np.zeros((n, 18), np.uint8)
+    li = np.full((n, 18), -1, np.int32)
+    ld = np.full((n, 18), -1.0, np.float32)
+    for d in range(1, 19):
+        nb = coords + np.array([CX[d], CY[d], CZ[d]])
+        inside = (nb[:, 0] >= 1) & (nb[:, 0] <= nx) & (nb[:, 1] >= 1) & (nb[:, 1] <= ny) & (nb[:, 2] == 1)
+        inlet = ~inside & (nb[:, 2] == 2) & (nb[:, 0] >= 1) & (nb[:, 0] <= nx) & (nb[:, 1] >= 1) & (nb[:, 1] <= ny)
+        wallm = ~inside & ~inlet
+        lt[wallm, d - 1] = H.LINK_WALL
+        ld[wallm, d - 1] = q
+        lt[inlet, d - 1] = H.LINK_INLET
+        li[inlet, d - 1] = 0
+        ld[inlet, d - 1] = 0.5
+    na = (lt == H.LINK_WALL).any(axis=1).astype(np.uint8)
+    nrm = np.tile(np.array([0.0, 0.0, 1.0], np.float32), (n, 1))
+    return H.Geometry((1, 1, 1), coords, lt, li, ld, na, nrm, [])
+
+
+def test_guo_zheng_shi_velocity_iolet_branch():
+    """GuoZhengShiDelegate.h:168-205: with a velocity iolet beyond the site,
+    the second estimate is the iolet's imposed velocity at x + c_i scaled by
+    (q - 1) / (q + 1); f_neq is not interpolated."""
+    q, tau, vmax, radius = 0.4, 0.8, 0.02, 10.0
+    pos, nrm = np.array([2.5, 2.5, 1.5]), np.array([0.0, 0.0, -1.0])
+    g = _wall_iolet_layer(q=q)
+    case = Case(g, wall_bc=H.WALL_GZS, inlet_bc=H.IOLET_LADD, tau=tau,
+                inlets=[IoletSpec(H.PARABOLIC, tuple(pos), tuple(nrm), radius=radius, max_speed=vmax)])
+    m = case.oracle()
+    f0 = 0.05 + 0.01 * anisotropic_f(m.n)
+    m.set_f(f0)
+    m.run(1)
+    got = m.f()
+    coords = m.lat.site_coords()
+    wall, iol, _, _ = m.lat.site_masks()
+    rho = f0.sum(axis=1)
+    mom = f0 @ C
+    omega = -1.0 / tau
+    checked = 0
+    for k in range(m.n):
+        fneq = f0[k] - helper_feq(rho[k], mom[k])
+        for d in range(1, 19):
+            i = INV[d]
+            if not ((wall[k] >> (d - 1)) & 1 and (iol[k] >> (i - 1)) & 1):
+                continue
+            x = coords[k] + C[i]
+            disp = x - pos
+            r2 = disp @ disp - (disp @ nrm) ** 2
+            v = nrm * vmax * (1.0 - r2 / radius ** 2)
+            u2 = v * (q - 1.0) / (q + 1.0)
+            m_w = q * (mom[k] * (1.0 - 1.0 / q)) + (1.0 - q) * rho[k] * u2
+            want = helper_feq(rho[k], m_w)[i] + (1.0 + omega) * fneq[i]
+            assert abs(got[k, i] - want) < 1e-10, (k, d)
+            checked += 1
+    assert checked >= m.n  # at least the straight -z wall link at every site
