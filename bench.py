@@ -8,13 +8,14 @@ parabolic velocity inlet driven by a sinusoidal peak-speed table (peak 0.05,
 period 4000) and two Nash pressure outlets at rho = 1, FP64 storage.
 
 The same workload is then timed in the FP32-shifted storage form (float
-g = f - w rho0, 19 x 4 B read + written + 18 pull indices = 224 B per site
-update), reported under "forms".
+g = f - w rho0, 19 x 4 B read + written = 152 B per site update), reported
+under "forms".
 
 Prints one JSON line (rank 0, N = 1).  MFLUPS = sites x steps / seconds / 1e6.
 The roofline object prices the step kernel against MI355X's 8 TB/s HBM with
 the algorithmic byte count per site update (FP64: 19 reads + 19 writes of 8 B
-plus 18 pull indices of 4 B = 376 B).  cpu_baseline times the CPU oracle (the
+= 304 B; the brick layout finds neighbours arithmetically, so no index table
+is read).  cpu_baseline times the CPU oracle (the
 reference's push/AoS FP64 algorithm, OpenMP over site ranges) on a bounded
 sample of the same workload.
 """
@@ -35,7 +36,7 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "tests"))
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_SITE = {"f64": 19 * 8 * 2 + 18 * 4, "f32": 19 * 4 * 2 + 18 * 4}
+BYTES_PER_SITE = {"f64": 19 * 8 * 2, "f32": 19 * 4 * 2}
 
 
 def log(msg: str) -> None:
@@ -109,14 +110,15 @@ def _counter_means(path: Path) -> dict:
     return {k: sums[k] / counts[k] for k in sums}
 
 
-def measure_traffic(workload: str) -> dict:
+def measure_traffic(workload: str, variant: int = 1) -> dict:
     """HBM bytes per step launch from two rocprofv3 --pmc passes, each scaled
     by the same counter's reading for the 2 GiB calibration copy.  Runs the
     profiled programs as child processes; this process never touches the GPU."""
     out = {}
     tmp = Path(os.environ.get("TMPDIR", "/tmp")) / f"hlb_pmc_{os.getpid()}"
     me = [sys.executable, "-u", str(Path(__file__).resolve())]
-    runs = {"bench": me + ["--steps", "10", "--warmup", "2", "--no-cpu-baseline", "--workload", workload],
+    runs = {"bench": me + ["--steps", "10", "--warmup", "2", "--no-cpu-baseline", "--no-ensemble", "--no-serving",
+                           "--workload", workload, "--variant", str(variant)],
             "cal": me + ["--calibrate"]}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         for tag, cmd in runs.items():
@@ -127,13 +129,24 @@ def measure_traffic(workload: str) -> dict:
                            stderr=subprocess.DEVNULL)
             out[(tag, counter)] = _counter_means(next(d.rglob("*counter_collection.csv")))
     res = {}
-    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-        cal = out[("cal", counter)]
-        copy_kib = max(v for k, v in cal.items() if "copy" in k.lower() or "elementwise" in k.lower())
-        factor = (CAL_BYTES / 1024.0) / copy_kib
-        step = [v for k, v in out[("bench", counter)].items() if "step_fp64" in k and "true" not in k.split("<")[1]]
-        res[counter] = {"raw_kib": max(step), "factor": factor, "bytes": max(step) * 1024.0 * factor}
-    res["bytes_per_launch"] = res["FETCH_SIZE"]["bytes"] + res["WRITE_SIZE"]["bytes"]
+    for kernel in ("step_fp64", "step_fp32"):
+        r = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            cal = out[("cal", counter)]
+            copy_kib = max(v for k, v in cal.items() if "copy" in k.lower() or "elementwise" in k.lower())
+            factor = (CAL_BYTES / 1024.0) / copy_kib
+            # the steady-state launches: not the first step (reads F), not a property step
+            # FP64: the template without the first-step / property flags; FP32's
+            # flags are run-time, its template arguments are the launch shape
+            def steady(name):
+                args = [x.strip() for x in name.split("<", 1)[1].split(">", 1)[0].split(",")]
+                return kernel == "step_fp32" or (args[1] == "false" and args[2] == "false")
+
+            step = [v for k, v in out[("bench", counter)].items() if kernel in k and steady(k)]
+            r[counter] = {"raw_kib": max(step), "factor": factor, "bytes": max(step) * 1024.0 * factor}
+        r["bytes_per_launch"] = r["FETCH_SIZE"]["bytes"] + r["WRITE_SIZE"]["bytes"]
+        res[kernel] = r
+    res["bytes_per_launch"] = res["step_fp64"]["bytes_per_launch"]
     return res
 
 
@@ -162,17 +175,21 @@ def time_steps(lat, steps: int):
     return ms_total
 
 
-def time_form(case, precision, steps: int, warmup: int):
+def time_form(case, precision, steps: int, warmup: int, variant=None):
     lat = case.product(precision=precision)
     lat.init_equilibrium(1.0)
     lat.step(max(1, warmup))
+    if variant is None:
+        variant, _ = lat.autotune(10)
+    else:
+        lat.set_variant(variant)
     lat.synchronize()
     ms = time_steps(lat, steps)
     lat.synchronize()
     stable = lat.stable()
     n = lat.n
     lat.close()
-    return n, ms, stable
+    return n, ms, stable, variant
 
 
 def percentile(xs, q):
@@ -345,6 +362,8 @@ def main() -> int:
     ap.add_argument("--serving-rate", type=float, default=8.0, help="Poisson arrivals per second")
     ap.add_argument("--serving-max-sites", type=float, default=5e6)
     ap.add_argument("--calibrate", action="store_true", help="run the 2 GiB calibration copy and exit")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="fixed launch shape instead of autotuning (used by the counter passes)")
     ap.add_argument("--measure-traffic", action="store_true",
                     help="measure HBM bytes per launch with rocprofv3 counters before the timed run")
     args = ap.parse_args()
@@ -352,7 +371,7 @@ def main() -> int:
         return calibrate_copy()
     traffic = None
     if args.measure_traffic:
-        traffic = measure_traffic(args.workload)
+        traffic = measure_traffic(args.workload, 1 if args.variant is None else args.variant)
         log(f"traffic: {json.dumps(traffic)}")
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -361,14 +380,19 @@ def main() -> int:
 
     import hemepure_amd as H
 
-    total = args.steps + args.warmup + 8
+    total = args.steps + args.warmup + 8 + 6 * 2 * 10 + 64  # + the launch-shape trials
     case, desc = make_case(args.workload, total)
     log(f"workload {args.workload}: {case.geom.n_sites} sites")
     lat = case.product()
     lat.init_equilibrium(1.0)
     lat.step(max(1, args.warmup))
+    if args.variant is None:
+        variant64, _ = lat.autotune(10)  # launch shape; the trial steps are real steps
+    else:
+        variant64 = args.variant
+        lat.set_variant(variant64)
     lat.synchronize()
-    log(f"warmup {args.warmup} steps done")
+    log(f"warmup {args.warmup} steps done, launch shape {variant64}")
 
     # Timed region: HIP events recorded on the stream the kernels run on.
     wall0 = time.perf_counter()
@@ -386,7 +410,7 @@ def main() -> int:
     log(f"{args.steps} steps in {ms_total:.2f} ms (host wall {wall*1e3:.2f} ms): {mflups:.1f} MFLUPS")
     lat.close()
 
-    n32, ms32, stable32 = time_form(case, H.FP32_SHIFTED, args.steps, args.warmup)
+    n32, ms32, stable32, variant32 = time_form(case, H.FP32_SHIFTED, args.steps, args.warmup, args.variant)
     mflups32 = n32 * args.steps / (ms32 / 1e3) / 1e6
     achieved32 = n32 * BYTES_PER_SITE["f32"] / (ms32 / 1e3 / args.steps) / 1e9
     log(f"fp32-shifted: {args.steps} steps in {ms32:.2f} ms: {mflups32:.1f} MFLUPS"
@@ -405,7 +429,7 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (procedurally generated seeded geometry)",
-        "config": {"workload": desc, "sites": n, "storage": "fp64 SoA, 32-bit pull table",
+        "config": {"workload": desc, "sites": n, "storage": "fp64 SoA in 8^3 bricks, arithmetic neighbour addressing",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -413,12 +437,13 @@ def main() -> int:
                      "bytes_per_site_update": BYTES_PER_SITE["f64"], "kernel": "step_fp64"},
     }
     out["forms"] = {
-        "fp64": {"mflups": round(mflups, 2), "ms_per_step": ms_total / args.steps,
+        "fp64": {"mflups": round(mflups, 2), "ms_per_step": ms_total / args.steps, "launch_variant": variant64,
                  "bytes_per_site_update": BYTES_PER_SITE["f64"], "hbm_frac": round(achieved / HBM_PEAK_GBS, 4)},
         "fp32_shifted": {"mflups": round(mflups32, 2), "ms_per_step": ms32 / args.steps,
                          "bytes_per_site_update": BYTES_PER_SITE["f32"],
                          "hbm_frac": round(achieved32 / HBM_PEAK_GBS, 4),
-                         "vs_fp64": round(mflups32 / mflups, 3), "kernel": "step_fp32"},
+                         "vs_fp64": round(mflups32 / mflups, 3), "kernel": "step_fp32",
+                         "launch_variant": variant32},
     }
     if not args.no_ensemble:
         out["ensemble"] = bench_ensemble(args.ensemble_members, args.ensemble_steps, 5000, args.workers)

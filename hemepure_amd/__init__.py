@@ -143,6 +143,9 @@ _SIGS = {
     "hlb_stability": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "hlb_iolet_state": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "hlb_monitor": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hlb_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
+    "hlb_variant": (C.c_int, [C.c_void_p]),
+    "hlb_autotune": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "hlb_set_params": (C.c_int, [C.c_void_p, C.POINTER(Params)]),
     "hlb_get_params": (C.c_int, [C.c_void_p, C.POINTER(Params)]),
     "hlb_set_iolets": (C.c_int, [C.c_void_p, C.POINTER(IoletDesc), C.c_int, C.POINTER(IoletDesc), C.c_int]),
@@ -467,6 +470,17 @@ class Lattice:
         s = C.c_int()
         _check(lib().hlb_stability(self._h, C.byref(s)))
         return bool(s.value)
+
+    def set_variant(self, variant: int):
+        _check(lib().hlb_set_variant(self._h, variant))
+
+    def variant(self) -> int:
+        return int(lib().hlb_variant(self._h))
+
+    def autotune(self, trial_steps=20):
+        v, ms = C.c_int(), C.c_double()
+        _check(lib().hlb_autotune(self._h, trial_steps, C.byref(v), C.byref(ms)))
+        return v.value, ms.value
 
     def monitor(self) -> dict:
         """Device reductions of the last property step: min/max density, max |u|, mass."""

@@ -773,8 +773,8 @@ Member32 member_record(const hlb_lattice* h, int props_at) {
 
 namespace {
 
-uint32_t member_grid(int64_t nslab) {
-  const int64_t per = fp32_variant_slabs(kFp32DefaultVariant);
+uint32_t member_grid(int64_t nslab, int variant) {
+  const int64_t per = fp32_variant_slabs(variant);
   const uint32_t tiles = (uint32_t)((nslab + per - 1) / per);
   return (tiles + 7) / 8 * 8;
 }
@@ -812,10 +812,10 @@ int enqueue_steps(hlb_lattice* h, int64_t n, bool props_last) {
   if (h->fp32()) {
     int rc = upload_member(h, props_last ? (int)(n - 1) : -1);
     if (rc != HLB_OK) return rc;
-    const uint32_t grid = member_grid(h->nslab);
+    const uint32_t grid = member_grid(h->nslab, h->variant);
     for (int64_t s = 0; s < n; ++s) {
       HLB_CHECK(launch_wk_begin(h->wk_refs.p, nwk, (int)s, h->stream));
-      HLB_CHECK(launch_step_fp32(h->mem.p, nullptr, grid, (int)s, kFp32DefaultVariant, h->stream));
+      HLB_CHECK(launch_step_fp32(h->mem.p, nullptr, grid, (int)s, h->variant, h->stream));
       HLB_CHECK(launch_gzs_fp32(h->mem.p, h->gz_links.p, (int)h->gz_links_host.size(), (int)s, h->stream));
       HLB_CHECK(launch_wk_end(h->wk_refs.p, nwk, h->stream));
     }
@@ -831,7 +831,7 @@ int enqueue_steps(hlb_lattice* h, int64_t n, bool props_last) {
     a.io_dens = h->io_dens.p + s * nio;
     a.io_vmax = h->io_vmax.p + s * nio;
     HLB_CHECK(launch_wk_begin(h->wk_refs.p, nwk, (int)s, h->stream));
-    HLB_CHECK(launch_step_fp64(a, h->params.kernel, first, props, h->stream));
+    HLB_CHECK(launch_step_fp64(a, h->params.kernel, first, props, h->variant, h->stream));
     HLB_CHECK(launch_gzs_fp64(a, h->params.kernel, h->gz_links.p, (int)h->gz_links_host.size(), a.io_vmax,
                               h->GW64.p, h->stream));
     HLB_CHECK(launch_wk_end(h->wk_refs.p, nwk, h->stream));
@@ -1037,6 +1037,38 @@ extern "C" int hlb_step_timed(hlb_handle h, int64_t nsteps, uint32_t flags, doub
   float elapsed = 0.0f;
   HLB_CHECK(hipEventElapsedTime(&elapsed, h->ev0, h->ev1));
   *ms = elapsed;
+  return HLB_OK;
+}
+
+extern "C" int hlb_set_variant(hlb_handle h, int variant) {
+  if (!h) return fail(HLB_EINVAL, "hlb_set_variant: null handle");
+  if (variant < 0 || variant >= kFp32Variants) return fail(HLB_EINVAL, "hlb_set_variant: no such variant");
+  if (h->device_ready) HLB_CHECK(hipStreamSynchronize(h->stream));
+  h->variant = variant;
+  return HLB_OK;
+}
+
+extern "C" int hlb_variant(hlb_handle h) { return h ? h->variant : -1; }
+
+extern "C" int hlb_autotune(hlb_handle h, int64_t trial_steps, int* variant, double* ms_per_step) {
+  if (!h) return fail(HLB_EINVAL, "hlb_autotune: null handle");
+  if (trial_steps < 1 || trial_steps > kChunk) return fail(HLB_EINVAL, "hlb_autotune: 1 to 1024 trial steps");
+  int best = h->variant;
+  double best_ms = 1e300;
+  for (int v = 0; v < kFp32Variants; ++v) {
+    h->variant = v;
+    double ms = 0.0;
+    int rc = hlb_step_timed(h, trial_steps, 0, &ms);  // the first launch of a shape warms it
+    if (rc == HLB_OK) rc = hlb_step_timed(h, trial_steps, 0, &ms);
+    if (rc != HLB_OK) return rc;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = v;
+    }
+  }
+  h->variant = best;
+  if (variant) *variant = best;
+  if (ms_per_step) *ms_per_step = best_ms / (double)trial_steps;
   return HLB_OK;
 }
 

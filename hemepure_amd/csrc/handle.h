@@ -15,7 +15,7 @@
 
 namespace hlb {
 
-hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, hipStream_t s);
+hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, int variant, hipStream_t s);
 hipError_t launch_gzs_fp64(const StepArgs& a, int kernel, const GzsLink* links, int n, const double* vmax,
                            double* GW, hipStream_t s);
 hipError_t launch_gzs_fp32(const Member32* members, const GzsLink* links, int n, int step, hipStream_t s);
@@ -152,6 +152,7 @@ struct hlb_lattice {
   std::vector<double> h_ladd_s;  // Ladd profile factors, 18 planes of SI, device iolet order
   bool device_ready = false;
   bool iolets_set = false;   // hlb_create with params, or hlb_set_iolets
+  int variant = hlb::kFp32DefaultVariant;  // launch shape of the steady-state step
 
   ~hlb_lattice() {
     if (stream) {

@@ -179,10 +179,12 @@ __device__ __forceinline__ uint32_t slab_record() {
   return xcd_tile() * (kBlock / kSlab) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 }
 
-template <int KERNEL, bool FIRST, bool PROPS>
-__global__ __launch_bounds__(kBlock) void step_fp64(StepArgs a) {
+// B threads (B / 64 slabs) per workgroup; REMAP: XCD-contiguous tile order.
+template <int KERNEL, bool FIRST, bool PROPS, int B = kBlock, bool REMAP = true>
+__global__ __launch_bounds__(B) void step_fp64(StepArgs a) {
   Lane L;
-  if (!slab_lane(a.bt, slab_record(), L)) return;
+  const uint32_t tile = REMAP ? xcd_tile() : blockIdx.x;
+  if (!slab_lane(a.bt, tile * (B / kSlab) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), L)) return;
   const uint32_t d = L.slot;
   const bool isB = L.boundary;
   uint32_t wm = 0, im = 0;
@@ -560,8 +562,35 @@ static unsigned slab_grid(int64_t nslab) {
   return (unsigned)((wg + kXcds - 1) / kXcds * kXcds);
 }
 
-hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, hipStream_t s) {
+// Steady-state steps in launch shape `variant` (as the FP32 kernel's); the
+// first step after an initial state and property steps use 256-thread tiles.
+template <int KN>
+void launch_variant_fp64(const StepArgs& a, int variant, hipStream_t s) {
+  const int per = fp32_variant_slabs(variant);
+  const int64_t tiles = (a.bt.nslab + per - 1) / per;
+  const dim3 grid((unsigned)((tiles + kXcds - 1) / kXcds * kXcds));
+#define HLB_V(B, R) hipLaunchKernelGGL((step_fp64<KN, false, false, B, R>), grid, dim3(B), 0, s, a)
+  switch (variant) {
+    case 0: HLB_V(128, false); break;
+    case 1: HLB_V(128, true); break;
+    case 2: HLB_V(256, false); break;
+    case 3: HLB_V(256, true); break;
+    case 4: HLB_V(512, false); break;
+    default: HLB_V(512, true); break;
+  }
+#undef HLB_V
+}
+
+hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, int variant, hipStream_t s) {
   if (a.bt.nslab <= 0) return hipSuccess;
+  if (!first && !props) {
+    switch (kernel) {
+      case HLB_KERNEL_LBGK: launch_variant_fp64<HLB_KERNEL_LBGK>(a, variant, s); break;
+      case HLB_KERNEL_TRT: launch_variant_fp64<HLB_KERNEL_TRT>(a, variant, s); break;
+      default: launch_variant_fp64<HLB_KERNEL_LES>(a, variant, s); break;
+    }
+    return hipGetLastError();
+  }
   const dim3 grid(slab_grid(a.bt.nslab)), block(kBlock);
 #define HLB_LAUNCH(KN, FI, PR) hipLaunchKernelGGL((step_fp64<KN, FI, PR>), grid, block, 0, s, a)
 #define HLB_KSEL(KN)                                   \

@@ -55,7 +55,7 @@ constexpr int kMaxIolets = 256;  // iolets per lattice
 // slabs (128, 256, 512 threads), each with and without the XCD-contiguous
 // tile order.
 constexpr int kFp32Variants = 6;
-constexpr int kFp32DefaultVariant = 3;  // 256 sites, XCD order
+constexpr int kFp32DefaultVariant = 1;  // 128 slots (two slabs), XCD order
 int fp32_variant_block(int variant);
 int fp32_variant_slabs(int variant);  // slabs per workgroup tile
 bool fp32_variant_remap(int variant);

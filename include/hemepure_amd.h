@@ -169,6 +169,14 @@ int hlb_upload_f(hlb_handle h, const double* f, int64_t t0);
  * during the last step (BaseStreamer::UpdateMinsAndMaxes,
  * src/lb/streamers/BaseStreamer.h:88-120).  Asynchronous. */
 int hlb_step(hlb_handle h, int64_t nsteps, uint32_t flags);
+/* Launch shape of the steady-state step (0..5: workgroups of 128/256/512
+ * sites-slots, without/with the XCD-contiguous order; default 3).  No
+ * counterpart in the reference.  hlb_autotune times every shape over
+ * trial_steps real steps of the lattice (they advance it) and keeps the
+ * fastest; results do not depend on the shape. */
+int hlb_set_variant(hlb_handle h, int variant);
+int hlb_variant(hlb_handle h);
+int hlb_autotune(hlb_handle h, int64_t trial_steps, int* variant, double* ms_per_step);
 /* Like hlb_step, timing the launches with HIP events recorded on the handle's
  * stream; blocks until done and returns the elapsed milliseconds. */
 int hlb_step_timed(hlb_handle h, int64_t nsteps, uint32_t flags, double* ms);

@@ -310,3 +310,23 @@ def test_guo_zheng_shi_fp32():
     ref.run(300, properties=True, threads=0)
     e = field_errors(lat, ref, "gzs cylinder fp32")
     assert e["u"] <= 1e-4 and e["wss"] <= 1e-4 and e["rho"] <= 1e-4
+
+
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32_SHIFTED])
+def test_launch_shapes_do_not_change_results(precision):
+    case = bifurcation_case(rp=8.0, lp=40.0, rd=6.5, ld=32.0, steps=100, peak=0.03, period=200.0)
+    ref = case.product(precision=precision)
+    ref.init_equilibrium(1.0)
+    ref.step(30, properties=True)
+    want = ref.download_f()
+    for v in range(6):
+        lat = case.product(precision=precision)
+        lat.set_variant(v)
+        lat.init_equilibrium(1.0)
+        lat.step(30, properties=True)
+        assert np.array_equal(lat.download_f(), want), f"variant {v}"
+    lat = case.product(precision=precision)
+    lat.init_equilibrium(1.0)
+    v, ms = lat.autotune(2)
+    lat.step(30 - 24, properties=True)
+    assert 0 <= v < 6 and ms > 0 and np.array_equal(lat.download_f(), want)
