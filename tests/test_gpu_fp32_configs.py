@@ -1,0 +1,51 @@
+"""FP32-shifted error against the FP64 path at the benchmark sizes (FP64 on
+the device is bit-identical to the oracle, so it stands in for it here):
+the Y-bifurcation (TRT, BFL, Ladd sinusoidal inlet) over 2000 steps and
+members of the stenosed-pipe ensemble over 5000 steps, each against its own
+FP64 run.  Bar: max |x32 - x64| / max |x64| <= 1e-4 for velocity and WSS, and
+for rho - 1."""
+import numpy as np
+import pytest
+
+import hemepure_amd as H
+from fixtures import bifurcation_case, ensemble_cases
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b, mask=None):
+    if mask is not None:
+        a, b = a[mask], b[mask]
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def run(case, precision, steps):
+    lat = case.product(precision=precision)
+    lat.init_equilibrium(1.0)
+    lat.step(steps, properties=True)
+    out = lat.fields()
+    lat.close()
+    return out
+
+
+def errors(case, steps, tag):
+    r64, u64, w64 = run(case, H.FP64, steps)
+    r32, u32, w32 = run(case, H.FP32_SHIFTED, steps)
+    wall = w64 < 1e300
+    e = {"rho": rel(r32 - 1, r64 - 1), "u": rel(u32, u64), "wss": rel(w32, w64, wall)}
+    print(f"[fp32 vs fp64] {tag}: " + ", ".join(f"{k} {v:.2e}" for k, v in e.items()), flush=True)
+    return e
+
+
+def test_bifurcation_full_size_2000_steps():
+    e = errors(bifurcation_case(steps=2100), 2000, "Y-bifurcation 708k sites, 2000 steps")
+    assert max(e.values()) <= 1e-4
+
+
+def test_ensemble_members_5000_steps():
+    worst = {"rho": 0.0, "u": 0.0, "wss": 0.0}
+    for i, case in enumerate(ensemble_cases(8, seed=2026, r_range=(6.0, 12.0), l_range=(60, 160))):
+        e = errors(case, 5000, f"ensemble member {i} ({case.geom.n_sites} sites, tau {case.tau:.3f})")
+        worst = {k: max(worst[k], e[k]) for k in worst}
+    print(f"[fp32 vs fp64] ensemble worst: {worst}", flush=True)
+    assert max(worst.values()) <= 1e-4
