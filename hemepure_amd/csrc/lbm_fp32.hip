@@ -477,17 +477,27 @@ hipError_t launch_cycle_delta(const double* u, float* prev, int64_t N, int64_t S
   return hipGetLastError();
 }
 
-int fp32_variant_block(int variant) { return variant / 2 == 0 ? 128 : (variant / 2 == 1 ? 256 : 512); }
+int fp32_variant_block(int variant) {
+  const int shape = variant % kFp32Shapes;
+  return shape / 2 == 0 ? 128 : (shape / 2 == 1 ? 256 : 512);
+}
 int fp32_variant_slabs(int variant) { return fp32_variant_block(variant) / kSlab; }
-bool fp32_variant_remap(int variant) { return (variant & 1) != 0; }
+bool fp32_variant_remap(int variant) { return ((variant % kFp32Shapes) & 1) != 0; }
+unsigned fp32_variant_lds(int variant) {
+  // 160 KB of LDS per CU, four SIMDs: w waves per SIMD <=> 160 KB / (4 w) per wave.
+  static const unsigned per_wave[4] = {0u, 10240u, 13653u, 20480u};
+  const int level = variant / kFp32Shapes;
+  return per_wave[level] * (unsigned)(fp32_variant_block(variant) / 64);
+}
 
 // `grid` workgroup tiles of the variant's size, padded to a multiple of the
 // XCD count (padding tiles map to member -1).
 hipError_t launch_step_fp32(const Member32* members, const int32_t* wg_member, uint32_t grid, int step,
                             int variant, hipStream_t s) {
   if (grid == 0) return hipSuccess;
-#define HLB_V(B, R) hipLaunchKernelGGL((step_fp32<B, R>), dim3(grid), dim3(B), 0, s, members, wg_member, step)
-  switch (variant) {
+  const unsigned lds = fp32_variant_lds(variant);
+#define HLB_V(B, R) hipLaunchKernelGGL((step_fp32<B, R>), dim3(grid), dim3(B), lds, s, members, wg_member, step)
+  switch (variant % kFp32Shapes) {
     case 0: HLB_V(128, false); break;
     case 1: HLB_V(128, true); break;
     case 2: HLB_V(256, false); break;

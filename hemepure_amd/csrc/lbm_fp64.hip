@@ -569,8 +569,9 @@ void launch_variant_fp64(const StepArgs& a, int variant, hipStream_t s) {
   const int per = fp32_variant_slabs(variant);
   const int64_t tiles = (a.bt.nslab + per - 1) / per;
   const dim3 grid((unsigned)((tiles + kXcds - 1) / kXcds * kXcds));
-#define HLB_V(B, R) hipLaunchKernelGGL((step_fp64<KN, false, false, B, R>), grid, dim3(B), 0, s, a)
-  switch (variant) {
+  const unsigned lds = fp32_variant_lds(variant);
+#define HLB_V(B, R) hipLaunchKernelGGL((step_fp64<KN, false, false, B, R>), grid, dim3(B), lds, s, a)
+  switch (variant % kFp32Shapes) {
     case 0: HLB_V(128, false); break;
     case 1: HLB_V(128, true); break;
     case 2: HLB_V(256, false); break;

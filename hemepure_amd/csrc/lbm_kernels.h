@@ -51,14 +51,19 @@ struct BrickTables {
 };
 constexpr int kMaxIolets = 256;  // iolets per lattice
 
-// Launch shapes of the FP32 member-table kernel: workgroup tile of 2, 4 or 8
-// slabs (128, 256, 512 threads), each with and without the XCD-contiguous
-// tile order.
-constexpr int kFp32Variants = 6;
-constexpr int kFp32DefaultVariant = 1;  // 128 slots (two slabs), XCD order
+// Launch shapes of the step kernels: variant = shape + 6 * level.  Shape
+// 0..5: workgroup tile of 2, 4 or 8 slabs (128, 256, 512 threads), each
+// without and with the XCD-contiguous tile order.  Level 0..3 caps the
+// waves resident per SIMD (none, 4, 3, 2) by giving each workgroup dynamic
+// LDS it does not use: fewer waves in flight keep more of each wave's
+// neighbour lines in the XCD's L2.
+constexpr int kFp32Shapes = 6;
+constexpr int kFp32Variants = 24;
+constexpr int kFp32DefaultVariant = 1;  // 128 slots (two slabs), XCD order, no cap
 int fp32_variant_block(int variant);
 int fp32_variant_slabs(int variant);  // slabs per workgroup tile
 bool fp32_variant_remap(int variant);
+unsigned fp32_variant_lds(int variant);  // dynamic LDS bytes per workgroup
 
 // Guo-Zheng-Shi wall links (GuoZhengShiDelegate.h:123-294).  Sites whose
 // pre-collision f the link kernel needs (GZS wall sites and the next fluid
