@@ -23,6 +23,8 @@ IOLET_NASH, IOLET_LADD = 0, 1
 COSINE, PARABOLIC, FILEVEL, WK2, WK3 = 0, 1, 2, 3, 4
 FP64, FP32_SHIFTED = 0, 1
 STEP_PROPERTIES = 1
+FIELD_PRESSURE, FIELD_VELOCITY, FIELD_SHEAR_STRESS, FIELD_DISTRIBUTIONS, FIELD_MPI_RANK = 0, 1, 2, 3, 4
+SELECT_WHOLE, SELECT_SURFACE, SELECT_INLET, SELECT_OUTLET, SELECT_PLANE, SELECT_SPHERE = 0, 1, 2, 3, 4, 5
 Q = 19
 NO_VALUE = np.finfo(np.float64).max
 
@@ -123,6 +125,13 @@ class RequestStatus(C.Structure):
     ]
 
 
+class XtrSpec(C.Structure):
+    _fields_ = [("fields", C.POINTER(C.c_int32)), ("names", C.POINTER(C.c_char_p)), ("n_fields", C.c_int32),
+                ("selector", C.c_int32),
+                ("voxel_size", C.c_double), ("time_step", C.c_double), ("origin", C.c_double * 3),
+                ("point", C.c_double * 3), ("normal", C.c_double * 3), ("radius", C.c_double)]
+
+
 REQ_QUEUED, REQ_RUNNING, REQ_CONVERGED, REQ_DONE, REQ_FAILED = 0, 1, 2, 3, 4
 
 _lib = None
@@ -171,6 +180,15 @@ _SIGS = {
     "hlb_batch_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "hlb_batch_variant": (C.c_int, [C.c_void_p]),
     "hlb_batch_autotune": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "hlb_xtr_open": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(XtrSpec), C.POINTER(C.c_void_p)]),
+    "hlb_xtr_capture": (C.c_int, [C.c_void_p]),
+    "hlb_xtr_write": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "hlb_xtr_write_fields": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p]),
+    "hlb_xtr_site_count": (C.c_int64, [C.c_void_p]),
+    "hlb_xtr_close": (None, [C.c_void_p]),
+    "hlb_checkpoint_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.POINTER(C.c_int64),
+                                      C.c_void_p]),
     "hlb_site_count": (C.c_int64, [C.c_void_p]),
     "hlb_class_counts": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hlb_site_coords": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -493,6 +511,76 @@ class Lattice:
         out = np.zeros(4, dtype=np.float64)
         _check(lib().hlb_iolet_state(self._h, int(bool(outlet)), int(index), _ptr(out)))
         return out
+
+
+    def checkpoint_read(self, path, target: int | None = None):
+        """(timestep, f) of a distributions checkpoint record; the last one when
+        target is None.  Restart with upload_f(f, timestep)."""
+        n = int(lib().hlb_site_count(self._h))
+        f = np.empty((n, Q), np.float64)
+        t = C.c_int64()
+        _check(lib().hlb_checkpoint_read(self._h, str(path).encode(), -1 if target is None else int(target),
+                                         C.byref(t), _ptr(f)))
+        return int(t.value), f
+
+
+class Extraction:
+    """A property-output file (.xtr, plus .off with distributions) of one lattice."""
+
+    def __init__(self, lat: "Lattice", path, fields, selector=SELECT_WHOLE, voxel_size=1.0, time_step=1.0,
+                 origin=(0.0, 0.0, 0.0), point=(0.0, 0.0, 0.0), normal=(0.0, 0.0, 1.0), radius=0.0, names=None):
+        self._lat = lat
+        self._fields = (C.c_int32 * len(fields))(*fields)
+        self._names = None
+        if names is not None:
+            self._names = (C.c_char_p * len(fields))(*[None if n is None else n.encode() for n in names])
+        spec = XtrSpec(C.cast(self._fields, C.POINTER(C.c_int32)),
+                       None if self._names is None else C.cast(self._names, C.POINTER(C.c_char_p)),
+                       len(fields), selector, voxel_size, time_step,
+                       (C.c_double * 3)(*origin), (C.c_double * 3)(*point), (C.c_double * 3)(*normal), radius)
+        h = C.c_void_p()
+        _check(lib().hlb_xtr_open(lat._h, str(path).encode(), C.byref(spec), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hlb_xtr_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def site_count(self) -> int:
+        return int(lib().hlb_xtr_site_count(self._h))
+
+    def capture(self):
+        _check(lib().hlb_xtr_capture(self._h))
+
+    def write(self) -> int:
+        t = C.c_int64()
+        _check(lib().hlb_xtr_write(self._h, C.byref(t)))
+        return int(t.value)
+
+    def write_fields(self, timestep: int, rho=None, u=None, wss=None, f=None):
+        def p(a, shape):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.float64)
+            assert a.size == int(np.prod(shape))
+            return a
+        n = int(lib().hlb_site_count(self._lat._h))
+        arrs = [p(rho, (n,)), p(u, (n, 3)), p(wss, (n,)), p(f, (n, Q))]
+        _check(lib().hlb_xtr_write_fields(self._h, int(timestep), *[None if a is None else _ptr(a) for a in arrs]))
 
 
 class Batch:

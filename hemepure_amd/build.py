@@ -19,7 +19,8 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libhemepure_amd.so"
 
-SOURCES = ["capi.hip", "batch.hip", "server.hip", "lbm_fp64.hip", "lbm_fp32.hip", "lattice.cpp", "geometry_gen.cpp", "gmy_reader.cpp"]
+SOURCES = ["capi.hip", "batch.hip", "server.hip", "lbm_fp64.hip", "lbm_fp32.hip", "lattice.cpp", "geometry_gen.cpp", "gmy_reader.cpp",
+           "extraction.cpp"]
 ARCH = os.environ.get("HLB_OFFLOAD_ARCH", "gfx950")
 
 

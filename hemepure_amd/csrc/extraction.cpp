@@ -1,0 +1,482 @@
+// extraction.cpp -- HemeLB extraction (.xtr + .off) writer and the
+// distribution-checkpoint reader, for one process.
+//
+// Formats (src/io/formats/extraction.h, offset.h), XDR big-endian:
+//   .xtr main header (60 B): HemeLB magic 0x686c6221, extraction magic
+//     0x78747204, version 4, voxel size (double, m), origin (3 doubles, m),
+//     site count (uint64), field count (uint32), field header length (uint32);
+//   field header: per field its name (XDR string), floats per site (uint32)
+//     and the offset subtracted before writing (double);
+//   one record per write: timestep (uint64), then per selected site in the
+//     lattice's site order: x, y, z (uint32) and each field's floats.
+//   .off (written when a field holds distributions): HemeLB magic, offset
+//     magic 0x6f666604, version 1, rank count, then the byte offset of each
+//     rank's part of a record and the end of the last (uint64).
+// LocalPropertyOutput (src/extraction/LocalPropertyOutput.cc:46-393) writes
+// these; LocalDistributionInput (LocalDistributionInput.cc:48-299) reads a
+// checkpoint back.  Values are converted as LbDataSourceIterator does
+// (LbDataSourceIterator.cc:37-95): pressure in mmHg, velocity in m/s, shear
+// stress in Pa, distributions in lattice units (the pre-collision state
+// entering the record's step).
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/hemepure_amd.h"
+#include "errors.h"
+
+namespace hlb {
+namespace {
+
+constexpr uint32_t kHemeLbMagic = 0x686c6221u;  // src/io/formats/formats.h:23
+constexpr uint32_t kXtrMagic = 0x78747204u;     // src/io/formats/extraction.h
+constexpr uint32_t kXtrVersion = 4u;
+constexpr uint32_t kOffMagic = 0x6f666604u;     // src/io/formats/offset.h
+constexpr uint32_t kOffVersion = 1u;
+constexpr size_t kMainHeader = 60;
+constexpr double kCs2 = 1.0 / 3.0;
+constexpr double kMmHgToPa = 133.3223874;       // src/constants.h:24
+constexpr double kBloodDensity = 1000.0;        // src/constants.h:25
+constexpr double kReferencePressure = 0.0;      // src/constants.h:23
+
+struct Xdr {
+  std::vector<unsigned char> b;
+  void u32(uint32_t v) {
+    for (int s = 24; s >= 0; s -= 8) b.push_back((unsigned char)(v >> s));
+  }
+  void u64(uint64_t v) {
+    u32((uint32_t)(v >> 32));
+    u32((uint32_t)v);
+  }
+  void f32(float v) {
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    u32(u);
+  }
+  void f64(double v) {
+    uint64_t u;
+    std::memcpy(&u, &v, 8);
+    u64(u);
+  }
+  void str(const std::string& s) {
+    u32((uint32_t)s.size());
+    b.insert(b.end(), s.begin(), s.end());
+    while (b.size() % 4) b.push_back(0);
+  }
+};
+
+struct XdrIn {
+  const unsigned char* p;
+  size_t n, at = 0;
+  bool ok = true;
+  uint32_t u32() {
+    if (at + 4 > n) {
+      ok = false;
+      return 0;
+    }
+    const uint32_t v = ((uint32_t)p[at] << 24) | ((uint32_t)p[at + 1] << 16) | ((uint32_t)p[at + 2] << 8) | p[at + 3];
+    at += 4;
+    return v;
+  }
+  uint64_t u64() {
+    const uint64_t hi = u32();
+    return (hi << 32) | u32();
+  }
+  float f32() {
+    const uint32_t u = u32();
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+  }
+  double f64() {
+    const uint64_t u = u64();
+    double d;
+    std::memcpy(&d, &u, 8);
+    return d;
+  }
+  std::string str() {
+    const uint32_t len = u32();
+    if (!ok || at + len > n) {
+      ok = false;
+      return {};
+    }
+    std::string s(reinterpret_cast<const char*>(p + at), len);
+    at += (len + 3) / 4 * 4;
+    return s;
+  }
+};
+
+uint32_t field_floats(int32_t f) {
+  switch (f) {
+    case HLB_FIELD_PRESSURE:
+    case HLB_FIELD_SHEAR_STRESS:
+    case HLB_FIELD_MPI_RANK: return 1;
+    case HLB_FIELD_VELOCITY: return 3;
+    case HLB_FIELD_DISTRIBUTIONS: return 19;
+    default: return 0;
+  }
+}
+
+const char* field_name(int32_t f) {
+  switch (f) {
+    case HLB_FIELD_PRESSURE: return "pressure";
+    case HLB_FIELD_VELOCITY: return "velocity";
+    case HLB_FIELD_SHEAR_STRESS: return "shearstress";
+    case HLB_FIELD_DISTRIBUTIONS: return "distributions";
+    default: return "mpirank";
+  }
+}
+
+// LocalPropertyOutput::GetOffset (LocalPropertyOutput.cc:395-403).
+double field_offset(int32_t f) { return f == HLB_FIELD_PRESSURE ? kReferencePressure : 0.0; }
+
+struct Vec3f {
+  float x, y, z;
+};
+inline float dot(Vec3f a, Vec3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline Vec3f sub(Vec3f a, Vec3f b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline Vec3f scale(Vec3f a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+// Vector3D<float>(location) * voxelSize + origin, evaluated in double and
+// stored as float (Vector3DArithmeticTraits<float, double> is double).
+inline Vec3f physical(const int64_t* g, double dx, const double* origin) {
+  return {(float)((double)(float)g[0] * dx + origin[0]), (float)((double)(float)g[1] * dx + origin[1]),
+          (float)((double)(float)g[2] * dx + origin[2])};
+}
+
+std::string off_path(const std::string& xtr) {
+  const size_t dot = xtr.rfind('.');
+  return dot == std::string::npos ? std::string() : xtr.substr(0, dot) + ".off";
+}
+
+int write_file(const std::string& path, const std::vector<unsigned char>& data, bool exclusive, const char* who) {
+  std::unique_ptr<FILE, int (*)(FILE*)> fh(std::fopen(path.c_str(), exclusive ? "wbx" : "ab"), &std::fclose);
+  if (!fh) return fail(HLB_EINVAL, std::string(who) + ": cannot create " + path + (exclusive ? " (it may exist)" : ""));
+  if (std::fwrite(data.data(), 1, data.size(), fh.get()) != data.size())
+    return fail(HLB_EINVAL, std::string(who) + ": short write to " + path);
+  return HLB_OK;
+}
+
+}  // namespace
+}  // namespace hlb
+
+using namespace hlb;
+
+struct hlb_xtr {
+  hlb_handle lattice = nullptr;
+  std::string path;
+  std::vector<int32_t> fields;
+  std::vector<std::string> names;
+  std::vector<int64_t> sites;  // selected sites, lattice order
+  std::vector<int64_t> coords;
+  double latticePressure = 0.0, latticeSpeed = 0.0;
+  bool wantFields = false, wantF = false;
+  int64_t captured = -1;  // step whose entering state f holds
+  std::vector<double> f;
+};
+
+extern "C" int hlb_xtr_open(hlb_handle h, const char* path, const hlb_xtr_spec* spec, hlb_xtr_handle* out) {
+  if (!h || !path || !spec || !out || (spec->n_fields > 0 && !spec->fields))
+    return fail(HLB_EINVAL, "hlb_xtr_open: null argument");
+  if (spec->n_fields < 1) return fail(HLB_EINVAL, "hlb_xtr_open: no fields");
+  if (!(spec->voxel_size > 0.0) || !(spec->time_step > 0.0))
+    return fail(HLB_EINVAL, "hlb_xtr_open: voxel size and time step must be positive");
+  try {
+    auto x = std::make_unique<hlb_xtr>();
+    x->lattice = h;
+    x->path = path;
+    bool dist = false;
+    for (int i = 0; i < spec->n_fields; ++i) {
+      if (field_floats(spec->fields[i]) == 0) return fail(HLB_EINVAL, "hlb_xtr_open: unsupported field");
+      dist = dist || spec->fields[i] == HLB_FIELD_DISTRIBUTIONS;
+      x->wantFields = x->wantFields || spec->fields[i] == HLB_FIELD_PRESSURE ||
+                      spec->fields[i] == HLB_FIELD_VELOCITY || spec->fields[i] == HLB_FIELD_SHEAR_STRESS;
+      x->fields.push_back(spec->fields[i]);
+      const char* nm = spec->names ? spec->names[i] : nullptr;
+      x->names.emplace_back(nm ? nm : field_name(spec->fields[i]));
+    }
+    // UnitConverter (src/util/UnitConverter.cc:15-24).
+    const double dx = spec->voxel_size, dt = spec->time_step;
+    x->latticePressure = kBloodDensity * dx * dx * dx / (dx * dt * dt);
+    x->latticeSpeed = dx / dt;
+    const int64_t N = hlb_site_count(h);
+    x->coords.resize((size_t)N * 3);
+    int rc = hlb_site_coords(h, x->coords.data());
+    if (rc != HLB_OK) return rc;
+    std::vector<uint32_t> wall(N), iol(N);
+    std::vector<int32_t> id(N), type(N);
+    rc = hlb_site_masks(h, wall.data(), iol.data(), id.data(), type.data());
+    if (rc != HLB_OK) return rc;
+    // Selector geometry in float, as the selectors hold it
+    // (PlaneGeometrySelector.h:30-60, SphereGeometrySelector.h:29-61).
+    Vec3f point{(float)spec->point[0], (float)spec->point[1], (float)spec->point[2]};
+    Vec3f normal{(float)spec->normal[0], (float)spec->normal[1], (float)spec->normal[2]};
+    if (spec->selector == HLB_SELECT_PLANE) {
+      const float mag = std::sqrt(dot(normal, normal));
+      if (!(mag > 0.0f)) return fail(HLB_EINVAL, "hlb_xtr_open: the plane normal is zero");
+      normal = {normal.x / mag, normal.y / mag, normal.z / mag};
+    }
+    const float radius = (float)spec->radius;
+    for (int64_t k = 0; k < N; ++k) {
+      bool in = true;
+      switch (spec->selector) {
+        case HLB_SELECT_WHOLE: break;                       // WholeGeometrySelector.cc
+        case HLB_SELECT_SURFACE: in = wall[k] != 0; break;  // GeometrySurfaceSelector.cc:13-17
+        case HLB_SELECT_INLET: in = type[k] == 2; break;    // InletSelector.cc:15-22, INLET_TYPE
+        case HLB_SELECT_OUTLET: in = type[k] == 3; break;   // OutletSelector.cc:15-22, OUTLET_TYPE
+        case HLB_SELECT_PLANE: {                            // PlaneGeometrySelector.cc:48-71
+          const Vec3f c = physical(&x->coords[3 * k], dx, spec->origin);
+          const float pd = dot(sub(c, point), normal);
+          if (std::abs(pd) > 0.5 * dx) {
+            in = false;
+          } else if (radius > 0.0) {
+            const Vec3f e = sub(sub(c, scale(normal, pd)), point);
+            in = dot(e, e) <= radius * radius;
+          }
+          break;
+        }
+        case HLB_SELECT_SPHERE: {                           // SphereGeometrySelector.cc:30-36
+          const Vec3f e = sub(physical(&x->coords[3 * k], dx, spec->origin), point);
+          in = std::sqrt(dot(e, e)) <= radius;
+          break;
+        }
+        default: return fail(HLB_EINVAL, "hlb_xtr_open: unknown selector");
+      }
+      if (in) x->sites.push_back(k);
+    }
+    uint64_t perSite = 12;
+    for (int32_t f : x->fields) perSite += 4ull * field_floats(f);
+    const uint64_t writeLength = 8 + perSite * (uint64_t)x->sites.size();
+    Xdr fh;
+    for (size_t i = 0; i < x->fields.size(); ++i) {
+      fh.str(x->names[i]);
+      fh.u32(field_floats(x->fields[i]));
+      fh.f64(field_offset(x->fields[i]));
+    }
+    Xdr mh;
+    mh.u32(kHemeLbMagic);
+    mh.u32(kXtrMagic);
+    mh.u32(kXtrVersion);
+    mh.f64(spec->voxel_size);
+    for (int a = 0; a < 3; ++a) mh.f64(spec->origin[a]);
+    mh.u64((uint64_t)x->sites.size());
+    mh.u32((uint32_t)x->fields.size());
+    mh.u32((uint32_t)fh.b.size());
+    mh.b.insert(mh.b.end(), fh.b.begin(), fh.b.end());
+    rc = write_file(x->path, mh.b, true, "hlb_xtr_open");
+    if (rc != HLB_OK) return rc;
+    if (dist) {
+      // LocalPropertyOutput::WriteOffsetFile (LocalPropertyOutput.cc:331-357), one rank.
+      const std::string op = off_path(x->path);
+      if (op.empty()) return fail(HLB_EINVAL, "hlb_xtr_open: the file name needs an extension for the .off file");
+      Xdr o;
+      o.u32(kHemeLbMagic);
+      o.u32(kOffMagic);
+      o.u32(kOffVersion);
+      o.u32(1);
+      o.u64(mh.b.size());
+      o.u64(mh.b.size() + writeLength);
+      rc = write_file(op, o.b, true, "hlb_xtr_open");
+      if (rc != HLB_OK) return rc;
+    }
+    x->wantF = dist;
+    *out = x.release();
+    return HLB_OK;
+  } catch (const std::bad_alloc&) {
+    return fail(HLB_EINVAL, "hlb_xtr_open: out of host memory");
+  }
+}
+
+extern "C" int hlb_xtr_write_fields(hlb_xtr_handle x, int64_t timestep, const double* rho, const double* u,
+                                    const double* wss, const double* f) {
+  if (!x) return fail(HLB_EINVAL, "hlb_xtr_write_fields: null handle");
+  for (int32_t fl : x->fields) {
+    const bool missing = (fl == HLB_FIELD_PRESSURE && !rho) || (fl == HLB_FIELD_VELOCITY && !u) ||
+                         (fl == HLB_FIELD_SHEAR_STRESS && !wss) || (fl == HLB_FIELD_DISTRIBUTIONS && !f);
+    if (missing) return fail(HLB_EINVAL, std::string("hlb_xtr_write_fields: no data for field ") + field_name(fl));
+  }
+  try {
+    Xdr w;
+    w.u64((uint64_t)timestep);
+    for (int64_t k : x->sites) {
+      for (int a = 0; a < 3; ++a) w.u32((uint32_t)x->coords[3 * k + a]);
+      for (int32_t fl : x->fields) {
+        switch (fl) {
+          case HLB_FIELD_PRESSURE: {
+            // ConvertPressureToPhysicalUnits(rho Cs2) - REFERENCE_PRESSURE_mmHg
+            const double p = kReferencePressure + (rho[k] * kCs2 - kCs2) * x->latticePressure / kMmHgToPa;
+            w.f32((float)(p - kReferencePressure));
+            break;
+          }
+          case HLB_FIELD_VELOCITY:
+            for (int a = 0; a < 3; ++a) w.f32((float)(u[3 * k + a] * x->latticeSpeed));
+            break;
+          case HLB_FIELD_SHEAR_STRESS: w.f32((float)(wss[k] * x->latticePressure)); break;
+          case HLB_FIELD_DISTRIBUTIONS:
+            for (int d = 0; d < 19; ++d) w.f32((float)f[k * 19 + d]);
+            break;
+          default: w.f32(0.0f); break;  // MpiRank: one process
+        }
+      }
+    }
+    return write_file(x->path, w.b, false, "hlb_xtr_write_fields");
+  } catch (const std::bad_alloc&) {
+    return fail(HLB_EINVAL, "hlb_xtr_write_fields: out of host memory");
+  }
+}
+
+// The reference writes record T at the end of step T (PropertyActor::
+// EndIteration, PropertyActor.cc:94-99) after LBM::EndIteration has swapped
+// the buffers (lb.hpp:336-346): the distributions it reads through GetFNew
+// are the ones that entered step T, and the density/velocity/stress caches
+// were filled by step T's collision from that same state.  hlb_xtr_capture
+// takes the state entering step T before it runs; hlb_xtr_write, after step T
+// ran with properties, writes record T.
+extern "C" int hlb_xtr_capture(hlb_xtr_handle x) {
+  if (!x) return fail(HLB_EINVAL, "hlb_xtr_capture: null handle");
+  x->captured = hlb_timestep(x->lattice);
+  if (!x->wantF) return HLB_OK;
+  try {
+    x->f.resize((size_t)hlb_site_count(x->lattice) * 19);
+  } catch (const std::bad_alloc&) {
+    return fail(HLB_EINVAL, "hlb_xtr_capture: out of host memory");
+  }
+  const int rc = hlb_download_f(x->lattice, x->f.data());
+  if (rc != HLB_OK) x->captured = -1;
+  return rc;
+}
+
+extern "C" int hlb_xtr_write(hlb_xtr_handle x, int64_t* timestep) {
+  if (!x) return fail(HLB_EINVAL, "hlb_xtr_write: null handle");
+  if (x->captured < 0) return fail(HLB_EINVAL, "hlb_xtr_write: no state captured (hlb_xtr_capture)");
+  const int64_t T = x->captured;
+  if (x->wantFields && hlb_timestep(x->lattice) != T + 1)
+    return fail(HLB_EINVAL, "hlb_xtr_write: fields need exactly the captured step run with HLB_STEP_PROPERTIES");
+  const int64_t N = hlb_site_count(x->lattice);
+  try {
+    std::vector<double> rho, u, wss;
+    if (x->wantFields) {
+      rho.resize(N);
+      u.resize(3 * N);
+      wss.resize(N);
+      const int rc = hlb_get_fields(x->lattice, rho.data(), u.data(), wss.data());
+      if (rc != HLB_OK) return rc;
+    }
+    const int rc = hlb_xtr_write_fields(x, T, x->wantFields ? rho.data() : nullptr, x->wantFields ? u.data() : nullptr,
+                                        x->wantFields ? wss.data() : nullptr, x->wantF ? x->f.data() : nullptr);
+    if (rc != HLB_OK) return rc;
+  } catch (const std::bad_alloc&) {
+    return fail(HLB_EINVAL, "hlb_xtr_write: out of host memory");
+  }
+  if (timestep) *timestep = T;
+  x->captured = -1;
+  return HLB_OK;
+}
+
+extern "C" int64_t hlb_xtr_site_count(hlb_xtr_handle x) { return x ? (int64_t)x->sites.size() : -1; }
+
+extern "C" void hlb_xtr_close(hlb_xtr_handle x) { delete x; }
+
+extern "C" int hlb_checkpoint_read(hlb_handle h, const char* path, int64_t target, int64_t* timestep, double* f) {
+  if (!h || !path || !timestep || !f) return fail(HLB_EINVAL, "hlb_checkpoint_read: null argument");
+  const std::string p(path), who("hlb_checkpoint_read");
+  std::vector<unsigned char> buf;
+  {
+    std::unique_ptr<FILE, int (*)(FILE*)> fh(std::fopen(path, "rb"), &std::fclose);
+    if (!fh) return fail(HLB_EINVAL, who + ": cannot open " + p);
+    std::fseek(fh.get(), 0, SEEK_END);
+    buf.resize((size_t)std::ftell(fh.get()));
+    std::fseek(fh.get(), 0, SEEK_SET);
+    if (std::fread(buf.data(), 1, buf.size(), fh.get()) != buf.size()) return fail(HLB_EINVAL, who + ": short read");
+  }
+  // LocalDistributionInput::ReadExtractionHeaders (LocalDistributionInput.cc:179-251).
+  XdrIn r{buf.data(), buf.size()};
+  if (r.u32() != kHemeLbMagic) return fail(HLB_EINVAL, who + ": This file does not start with the HemeLB magic number.");
+  if (r.u32() != kXtrMagic) return fail(HLB_EINVAL, who + ": This file does not have the extraction magic number.");
+  if (r.u32() != kXtrVersion) return fail(HLB_EINVAL, who + ": Version number incorrect.");
+  (void)r.f64();
+  for (int a = 0; a < 3; ++a) (void)r.f64();
+  const uint64_t nSites = r.u64();
+  const uint32_t nFields = r.u32(), fieldHeaderLength = r.u32();
+  if (nFields != 1)
+    return fail(HLB_EINVAL, who + ": Checkpoint file must contain exactly one field, the distributions, but has " +
+                                std::to_string(nFields));
+  if (fieldHeaderLength != 32)
+    return fail(HLB_EINVAL, who + ": Checkpoint file's field header must be 32 B long");
+  const std::string name = r.str();
+  const uint32_t nFloats = r.u32();
+  const double offset = r.f64();
+  if (!r.ok || name != "distributions")
+    return fail(HLB_EINVAL, who + ": Checkpoint file must contain field named 'distributions', but has '" + name + "'");
+  if (nFloats != 19) return fail(HLB_EINVAL, who + ": the distributions field must hold 19 floats per site");
+  const uint64_t headerLength = kMainHeader + 32;
+  // LocalDistributionInput::ReadOffsets (:253-299).
+  std::vector<unsigned char> ob;
+  {
+    const std::string op = off_path(p);
+    std::unique_ptr<FILE, int (*)(FILE*)> fh(std::fopen(op.c_str(), "rb"), &std::fclose);
+    if (!fh) return fail(HLB_EINVAL, who + ": cannot open the offset file " + op);
+    std::fseek(fh.get(), 0, SEEK_END);
+    ob.resize((size_t)std::ftell(fh.get()));
+    std::fseek(fh.get(), 0, SEEK_SET);
+    if (std::fread(ob.data(), 1, ob.size(), fh.get()) != ob.size()) return fail(HLB_EINVAL, who + ": short read");
+  }
+  XdrIn o{ob.data(), ob.size()};
+  if (o.u32() != kHemeLbMagic || o.u32() != kOffMagic || o.u32() != kOffVersion)
+    return fail(HLB_EINVAL, who + ": not an offset file of version 1");
+  const uint32_t nRanks = o.u32();
+  std::vector<uint64_t> offs(nRanks + 1);
+  for (auto& v : offs) v = o.u64();
+  if (!o.ok || nRanks == 0) return fail(HLB_EINVAL, who + ": truncated offset file");
+  const uint64_t allLength = offs[nRanks] - offs[0];
+  if (buf.size() < headerLength || allLength == 0 || (buf.size() - headerLength) % allLength)
+    return fail(HLB_EINVAL, who + ": Checkpoint file length not consistent with integer number of checkpoints");
+  const uint64_t nTimes = (buf.size() - headerLength) / allLength;
+  auto timeAt = [&](uint64_t i) {
+    XdrIn t{buf.data() + offs[0] + i * allLength, 8};
+    return t.u64();
+  };
+  uint64_t iTS = nTimes - 1, ts = timeAt(iTS);
+  if (target >= 0) {  // binary search, as the reference (:82-99)
+    uint64_t lo = 0, len = nTimes;
+    while (len != 0) {
+      const uint64_t half = len / 2, m = lo + half;
+      ts = timeAt(m);
+      if (ts < (uint64_t)target) {
+        lo = m + 1;
+        len -= half + 1;
+      } else {
+        len = half;
+      }
+    }
+    iTS = lo;
+    if (iTS >= nTimes || (ts = timeAt(iTS)) != (uint64_t)target)
+      return fail(HLB_EINVAL, who + ": Target timestep " + std::to_string(target) + " not found in checkpoint file.");
+  }
+  const int64_t N = hlb_site_count(h);
+  if (nSites != (uint64_t)N)
+    return fail(HLB_EINVAL, who + ": the checkpoint holds " + std::to_string(nSites) + " sites, the lattice " +
+                                std::to_string(N));
+  std::vector<int64_t> coords((size_t)N * 3);
+  int rc = hlb_site_coords(h, coords.data());
+  if (rc != HLB_OK) return rc;
+  XdrIn d{buf.data() + offs[0] + iTS * allLength, allLength};
+  *timestep = (int64_t)d.u64();
+  for (int64_t k = 0; k < N; ++k) {
+    for (int a = 0; a < 3; ++a)
+      if ((int64_t)d.u32() != coords[3 * k + a])
+        return fail(HLB_EINVAL, who + ": Site read at index " + std::to_string(k) + " is not the lattice's site " +
+                                    std::to_string(k));
+    for (int q = 0; q < 19; ++q) {
+      float v = d.f32();
+      v += offset;  // float += double, as LocalDistributionInput.cc:153-156
+      f[k * 19 + q] = (double)v;
+    }
+  }
+  if (!d.ok) return fail(HLB_EINVAL, who + ": truncated checkpoint record");
+  return HLB_OK;
+}

@@ -185,9 +185,9 @@ int hlb_step_timed(hlb_handle h, int64_t nsteps, uint32_t flags, double* ms);
 int hlb_synchronize(hlb_handle h);
 
 /* ---- extraction / checkpoint ------------------------------------------- */
-/* LbDataSourceIterator::GetDistribution after the swap
- * (src/extraction/LbDataSourceIterator.cc:90-93): the pre-collision f that
- * enters the next step, AoS f[site*19+d] in the reference's site order. */
+/* LatticeData::GetFOld after LBM::EndIteration's swap (src/lb/lb.hpp:336-346):
+ * the pre-collision f entering step hlb_timestep(h), AoS f[site*19+d] in the
+ * reference's site order. */
 int hlb_download_f(hlb_handle h, double* f);
 /* MacroscopicPropertyCache density/velocity/WSS of the last step run with
  * HLB_STEP_PROPERTIES (src/lb/MacroscopicPropertyCache.h:24-106).  u is n*3;
@@ -199,6 +199,70 @@ int64_t hlb_timestep(hlb_handle h);
 /* StabilityTester (src/lb/StabilityTester.h:100-117): 1 if every
  * distribution entering the next step is > 0 (and not NaN). */
 int hlb_stability(hlb_handle h, int* stable);
+
+/* ---- property extraction files (.xtr + .off) ---------------------------- */
+/* Field types of OutputField (src/extraction/OutputField.h).  A field is
+ * written under its given name, or its type name ("pressure", "velocity",
+ * "shearstress", "distributions", "mpirank") when none is given, as
+ * SimConfig::DoIOForPropertyField (SimConfig.cc:565-580) defaults it. */
+#define HLB_FIELD_PRESSURE 0      /* mmHg, offset REFERENCE_PRESSURE_mmHg */
+#define HLB_FIELD_VELOCITY 1      /* m/s, 3 floats */
+#define HLB_FIELD_SHEAR_STRESS 2  /* Pa, wall sites; elsewhere inf */
+#define HLB_FIELD_DISTRIBUTIONS 3 /* 19 floats, lattice units */
+#define HLB_FIELD_MPI_RANK 4      /* 0: one process */
+/* Geometry selectors (the src/extraction/...Selector.cc classes). */
+#define HLB_SELECT_WHOLE 0
+#define HLB_SELECT_SURFACE 1 /* wall sites */
+#define HLB_SELECT_INLET 2   /* SiteType INLET_TYPE */
+#define HLB_SELECT_OUTLET 3  /* SiteType OUTLET_TYPE */
+#define HLB_SELECT_PLANE 4   /* |distance| <= voxel/2, within radius if > 0 */
+#define HLB_SELECT_SPHERE 5  /* within radius of point */
+/* One <propertyoutput> of the XML (SimConfig::DoIOForPropertyOutputFile,
+ * src/configuration/SimConfig.cc:420-560): the fields, the
+ * selector with its point (m), normal and radius (m), and the unit
+ * conversion (voxel size in m, time step in s, origin in m). */
+typedef struct hlb_xtr_spec {
+  const int32_t* fields;
+  const char* const* names; /* n_fields names, or NULL; a NULL entry: type name */
+  int32_t n_fields;
+  int32_t selector;
+  double voxel_size;
+  double time_step;
+  double origin[3];
+  double point[3];
+  double normal[3];
+  double radius;
+} hlb_xtr_spec;
+typedef struct hlb_xtr* hlb_xtr_handle;
+/* LocalPropertyOutput::LocalPropertyOutput (src/extraction/
+ * LocalPropertyOutput.cc:27-198): selects the sites, creates the file (it
+ * must not exist) and writes the headers; with a distributions field also the
+ * .off file (WriteOffsetFile, :331-357). */
+int hlb_xtr_open(hlb_handle h, const char* path, const hlb_xtr_spec* spec, hlb_xtr_handle* out);
+/* Record T holds what the reference writes at the end of step T
+ * (PropertyActor::EndIteration, src/extraction/PropertyActor.cc:94-99): the
+ * distributions that entered step T and step T's property caches.  Call
+ * hlb_xtr_capture before step T (it copies f when a field needs it), run
+ * step T with HLB_STEP_PROPERTIES, then hlb_xtr_write (LocalPropertyOutput::
+ * Write, :216-327), which returns T.  A distributions-only record needs no
+ * step in between. */
+int hlb_xtr_capture(hlb_xtr_handle x);
+int hlb_xtr_write(hlb_xtr_handle x, int64_t* timestep);
+/* Writes one record from caller arrays in the lattice's site order (rho n,
+ * u n*3, wss n, f n*19, lattice units); arrays a field does not use may be
+ * NULL. */
+int hlb_xtr_write_fields(hlb_xtr_handle x, int64_t timestep, const double* rho, const double* u,
+                         const double* wss, const double* f);
+int64_t hlb_xtr_site_count(hlb_xtr_handle x);
+void hlb_xtr_close(hlb_xtr_handle x);
+/* LocalDistributionInput::LoadDistribution (src/extraction/
+ * LocalDistributionInput.cc:48-177): reads the record of timestep `target`
+ * (the last record when target < 0) of a distributions-only .xtr and its .off
+ * file, checks every site's coordinates against the lattice, and returns the
+ * timestep and f (n*19, float values widened).  Pass both to hlb_upload_f to
+ * restart: CheckpointInitialCondition (src/lb/InitialCondition.hpp:46-50,
+ * InitialCondition.cc:18-21) sets the next step to the record's. */
+int hlb_checkpoint_read(hlb_handle h, const char* path, int64_t target, int64_t* timestep, double* f);
 
 /* ---- batches of FP32-shifted lattices ----------------------------------- */
 /* No counterpart in the reference, which runs one lattice per process: the

@@ -408,10 +408,109 @@ class LBM {
 
 }  // namespace lb
 
+namespace extraction {
+
+/* OutputField (src/extraction/OutputField.h): a type and the name written to
+ * the file (defaults to the type name). */
+struct OutputField {
+  enum FieldType {
+    Pressure = HLB_FIELD_PRESSURE,
+    Velocity = HLB_FIELD_VELOCITY,
+    ShearStress = HLB_FIELD_SHEAR_STRESS,
+    Distributions = HLB_FIELD_DISTRIBUTIONS,
+    MpiRank = HLB_FIELD_MPI_RANK
+  };
+  FieldType type;
+  std::string name;
+};
+
+/* PropertyOutputFile (src/extraction/PropertyOutputFile.h) with its geometry
+ * selector: which sites, which fields, written every `frequency` steps from
+ * `start` to `stop`. */
+struct PropertyOutputFile {
+  std::string filename;
+  uint64_t frequency = 1, start = 0, stop = UINT64_MAX;
+  std::vector<OutputField> fields;
+  int32_t selector = HLB_SELECT_WHOLE;
+  double point[3] = {0, 0, 0}, normal[3] = {0, 0, 1}, radius = 0;
+};
+
+/* LocalPropertyOutput (src/extraction/LocalPropertyOutput.cc): opening
+ * writes the headers; ShouldWrite as :205-210; Capture before step T, Write
+ * after it (hlb_xtr_capture / hlb_xtr_write). */
+class LocalPropertyOutput {
+ public:
+  LocalPropertyOutput(geometry::LatticeData& latDat, const PropertyOutputFile& spec, PhysicalDistance voxelSize,
+                      PhysicalTime timeStep, const double origin[3])
+      : outputSpec(spec) {
+    std::vector<int32_t> types;
+    std::vector<const char*> names;
+    for (const auto& f : spec.fields) {
+      types.push_back(f.type);
+      names.push_back(f.name.empty() ? nullptr : f.name.c_str());
+    }
+    hlb_xtr_spec x{};
+    x.fields = types.data();
+    x.names = names.data();
+    x.n_fields = (int32_t)types.size();
+    x.selector = spec.selector;
+    x.voxel_size = voxelSize;
+    x.time_step = timeStep;
+    for (int a = 0; a < 3; ++a) {
+      x.origin[a] = origin[a];
+      x.point[a] = spec.point[a];
+      x.normal[a] = spec.normal[a];
+    }
+    x.radius = spec.radius;
+    hlb_xtr_handle h = nullptr;
+    detail::Check(hlb_xtr_open(latDat.Handle(), spec.filename.c_str(), &x, &h));
+    handle.reset(h);
+    for (const auto& f : spec.fields) needsProperties = needsProperties || f.type != OutputField::Distributions;
+  }
+  bool ShouldWrite(uint64_t t) const {
+    return t >= outputSpec.start && t <= outputSpec.stop && t % outputSpec.frequency == 0;
+  }
+  bool NeedsProperties() const { return needsProperties; }
+  void Capture() { detail::Check(hlb_xtr_capture(handle.get())); }
+  int64_t Write() {
+    int64_t t = -1;
+    detail::Check(hlb_xtr_write(handle.get(), &t));
+    return t;
+  }
+  const PropertyOutputFile* GetOutputSpec() const { return &outputSpec; }
+
+ private:
+  struct Closer {
+    void operator()(hlb_xtr* x) const { hlb_xtr_close(x); }
+  };
+  PropertyOutputFile outputSpec;
+  std::unique_ptr<hlb_xtr, Closer> handle;
+  bool needsProperties = false;
+};
+
+/* LocalDistributionInput::LoadDistribution (LocalDistributionInput.cc:48-177):
+ * the record of `target` (the last when target < 0) of a checkpoint file;
+ * returns its timestep and fills f (AoS, site order). */
+inline int64_t LoadDistribution(geometry::LatticeData& latDat, const std::string& path, int64_t target,
+                                std::vector<distribn_t>& f) {
+  f.resize((size_t)latDat.GetLocalFluidSiteCount() * 19);
+  int64_t t = 0;
+  detail::Check(hlb_checkpoint_read(latDat.Handle(), path.c_str(), target, &t, f.data()));
+  return t;
+}
+
+}  // namespace extraction
+
 /* SimulationMaster (src/SimulationMaster.cc): owns the geometry, lattice and
  * LBM of one run and advances them; property caches are refreshed every
  * propertyPeriod steps and stability is checked every stabilityPeriod steps
- * (an unstable state throws, where the reference aborts). */
+ * (an unstable state throws, where the reference aborts).  Property output
+ * files are written at the end of each step they are due, as PropertyActor
+ * does (PropertyActor.cc:94-99).  A checkpoint file as initial condition
+ * (CheckpointInitialCondition, InitialCondition.hpp:46-50) loads the record
+ * of checkpointTime (the last record when negative); the step counter takes
+ * the record's timestep only when checkpointTime is given, as
+ * InitialConditionBase::SetTime does (InitialCondition.cc:18-21). */
 template <class LBMType>
 class SimulationMaster {
  public:
@@ -424,21 +523,46 @@ class SimulationMaster {
     int64_t propertyPeriod = 0;
     int64_t stabilityPeriod = 0;
     int32_t storage = HLB_FP64;
+    /* unit conversion of the output files (SimConfig: voxel size, time step,
+     * geometry origin) */
+    PhysicalDistance voxelSize = 1.0;
+    PhysicalTime timeStepLength = 1.0;
+    double origin[3] = {0, 0, 0};
+    std::vector<extraction::PropertyOutputFile> propertyOutputs;
+    std::string checkpointFile;
+    int64_t checkpointTime = -1;
   };
 
   explicit SimulationMaster(const Config& c)
       : config(c), latticeData(c.geometry), lbm(latticeData, c.parameters, c.inlets, c.outlets, c.storage) {
-    lbm.SetInitialConditions(c.initialDensity, 1);
+    if (c.checkpointFile.empty()) {
+      lbm.SetInitialConditions(c.initialDensity, 1);
+    } else {
+      std::vector<distribn_t> f;
+      const int64_t t = extraction::LoadDistribution(latticeData, c.checkpointFile, c.checkpointTime, f);
+      lbm.SetInitialConditions(f, c.checkpointTime >= 0 ? t : 1);
+    }
+    for (const auto& spec : c.propertyOutputs)
+      outputs.emplace_back(new extraction::LocalPropertyOutput(latticeData, spec, c.voxelSize, c.timeStepLength,
+                                                               c.origin));
   }
 
   void DoTimeStep() {
     const int64_t t = lbm.GetTimeStep();
     if (config.propertyPeriod > 0 && t % config.propertyPeriod == 0) lbm.RequestPropertyRefresh();
+    std::vector<extraction::LocalPropertyOutput*> due;
+    for (auto& o : outputs)
+      if (o->ShouldWrite((uint64_t)t)) {
+        o->Capture();
+        if (o->NeedsProperties()) lbm.RequestPropertyRefresh();
+        due.push_back(o.get());
+      }
     lbm.RequestComms();
     lbm.PreSend();
     lbm.PreReceive();
     lbm.PostReceive();
     lbm.EndIteration();
+    for (auto* o : due) o->Write();
     if (config.stabilityPeriod > 0 && t % config.stabilityPeriod == 0 && !lbm.IsStable())
       throw Exception("SimulationMaster: the simulation became unstable at step " + std::to_string(t));
   }
@@ -449,7 +573,9 @@ class SimulationMaster {
     while (lbm.GetTimeStep() <= config.totalSteps) {
       const int64_t t = lbm.GetTimeStep();
       int64_t run = config.totalSteps - t + 1;
-      for (int64_t period : {config.propertyPeriod, config.stabilityPeriod})
+      std::vector<int64_t> periods{config.propertyPeriod, config.stabilityPeriod};
+      for (const auto& o : outputs) periods.push_back((int64_t)o->GetOutputSpec()->frequency);
+      for (int64_t period : periods)
         if (period > 0) run = std::min(run, period - (t % period == 0 ? period : t % period) + 1);
       if (run > 1) {
         lbm.Iterate(run - 1);
@@ -466,6 +592,7 @@ class SimulationMaster {
   Config config;
   geometry::LatticeData latticeData;
   LBMType lbm;
+  std::vector<std::unique_ptr<extraction::LocalPropertyOutput>> outputs;
 };
 
 }  // namespace hemelb

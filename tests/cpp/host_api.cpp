@@ -1,7 +1,9 @@
 // Exercises the C++ host API (include/hemepure_amd.hpp) the way reference
 // code would: LatticeData accessors, then (mode "run") an LBM with TRT + BFL
-// selected by type and a SimulationMaster run.  Prints one line per fact for
-// the test to compare with the Python binding.
+// selected by type and a SimulationMaster run, or (mode "extract") a run
+// writing a checkpoint file and a plane of pressure and velocity, then a
+// second run restarted from the checkpoint.  Prints one line per fact for the
+// test to compare with the Python binding.
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -43,6 +45,42 @@ int main(int argc, char** argv) {
     }
     SimulationMaster<TrtBflNash>::Config cfg{g, lb::LbmParameters(0.7), {lb::iolets::InOutLetCosine(pin, nin, 1.002)},
                                              {lb::iolets::InOutLetCosine(pout, nout, 1.0)}};
+    if (mode == "extract") {
+      const std::string dir = argv[2];
+      cfg.totalSteps = 150;
+      cfg.voxelSize = 1e-4;
+      cfg.timeStepLength = 1e-5;
+      extraction::PropertyOutputFile cp;
+      cp.filename = dir + "/cp.xtr";
+      cp.frequency = 50;
+      cp.fields = {{extraction::OutputField::Distributions, ""}};
+      extraction::PropertyOutputFile flow;
+      flow.filename = dir + "/flow.xtr";
+      flow.frequency = 75;
+      flow.fields = {{extraction::OutputField::Pressure, "p"}, {extraction::OutputField::Velocity, ""}};
+      flow.selector = HLB_SELECT_PLANE;
+      flow.point[2] = 12 * cfg.voxelSize;
+      cfg.propertyOutputs = {cp, flow};
+      {
+        SimulationMaster<TrtBflNash> master(cfg);
+        master.RunSimulation();
+        std::printf("timestep %lld\n", (long long)master.GetLBM().GetTimeStep());
+      }
+      auto again = cfg;
+      again.propertyOutputs.clear();
+      again.checkpointFile = dir + "/cp.xtr";
+      again.checkpointTime = 100;
+      SimulationMaster<TrtBflNash> restarted(again);
+      std::printf("restart %lld\n", (long long)restarted.GetLBM().GetTimeStep());
+      restarted.RunSimulation();
+      std::printf("final %lld\n", (long long)restarted.GetLBM().GetTimeStep());
+      const auto f = restarted.GetLBM().GetDistributions();
+      FILE* out = std::fopen((dir + "/restart.bin").c_str(), "wb");
+      if (!out) return 4;
+      std::fwrite(f.data(), sizeof(double), f.size(), out);
+      std::fclose(out);
+      return 0;
+    }
     cfg.totalSteps = 150;
     cfg.propertyPeriod = 150;
     cfg.stabilityPeriod = 50;

@@ -69,3 +69,47 @@ def test_simulation_master_matches_the_python_binding(tmp_path):
     assert np.array_equal(raw[n:4 * n], u.ravel())
     assert np.array_equal(raw[4 * n:], f.ravel())
     assert got["stable"] == ["1"]
+
+
+@pytest.mark.gpu
+def test_simulation_master_extraction_and_checkpoint_restart(tmp_path):
+    """SimulationMaster with two property outputs (a checkpoint every 50 steps,
+    pressure and velocity on the plane z = 12 every 75) and a second run
+    restarted from the checkpoint of step 100: the files are byte-identical
+    to the ones the Python binding writes for the same run, and the restart
+    ends on the distributions of the same restart made through the binding."""
+    got = {l[0]: l[1:] for l in run("extract", tmp_path)}
+    assert got["timestep"] == ["151"] and got["restart"] == ["100"] and got["final"] == ["151"]
+    g = H.Geometry.cylinder(6.0, 24, 16)
+    i0, o0 = g.inlets()[0], g.outlets()[0]
+
+    def lattice():
+        return H.Lattice(g, H.params(kernel=H.KERNEL_TRT, wall_bc=H.WALL_BFL, tau=0.7),
+                         [H.iolet_desc(H.COSINE, i0["position"], i0["normal"], density_mean=1.002)],
+                         [H.iolet_desc(H.COSINE, o0["position"], o0["normal"], density_mean=1.0)])
+
+    lat = lattice()
+    lat.init_equilibrium(1.0)
+    ref = tmp_path / "ref"
+    ref.mkdir()
+    cp = H.Extraction(lat, ref / "cp.xtr", [H.FIELD_DISTRIBUTIONS], voxel_size=1e-4, time_step=1e-5)
+    flow = H.Extraction(lat, ref / "flow.xtr", [H.FIELD_PRESSURE, H.FIELD_VELOCITY], names=["p", None],
+                        selector=H.SELECT_PLANE, point=(0.0, 0.0, 12 * 1e-4), normal=(0.0, 0.0, 1.0),
+                        voxel_size=1e-4, time_step=1e-5)
+    for t in range(1, 151):
+        due = [x for x, period in ((cp, 50), (flow, 75)) if t % period == 0]
+        for x in due:
+            x.capture()
+        lat.step(1, properties=flow in due)
+        for x in due:
+            assert x.write() == t
+    cp.close()
+    flow.close()
+    for name in ("cp.xtr", "cp.off", "flow.xtr"):
+        assert (tmp_path / name).read_bytes() == (ref / name).read_bytes(), name
+    again = lattice()
+    t, f = again.checkpoint_read(ref / "cp.xtr", 100)
+    again.upload_f(f, t)
+    again.step(151 - t)
+    raw = np.fromfile(tmp_path / "restart.bin", dtype=np.float64)
+    assert np.array_equal(raw, again.download_f().ravel())

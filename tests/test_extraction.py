@@ -1,0 +1,323 @@
+"""Property extraction (.xtr/.off) writer and checkpoint reader, on the CPU.
+
+The writer runs on host arrays (hlb_xtr_write_fields), so these tests need no
+GPU.  The file is parsed by an independent reader written from the format
+(src/io/formats/extraction.h, offset.h); the main and field headers are
+checked byte for byte against the reference's own unit test
+(src/unittests/extraction/LocalPropertyOutputTests.h:97-141), which writes a
+4x4x4 cube with voxel 0.3 mm, origin (0.034, 0.001, 0.074) m and fields
+"Pressure" and "Velocity".
+"""
+from __future__ import annotations
+
+import struct
+
+import numpy as np
+import pytest
+
+import hemepure_amd as H
+from fixtures import CX, CY, CZ, closed_box, four_cube
+
+MMHG = 133.3223874
+CS2 = 1.0 / 3.0
+
+# LocalPropertyOutputTests.h:108-122 and :128-139.
+GOLDEN_MAIN_HEADER = bytes.fromhex(
+    "686C6221" "78747204" "00000004" "3F33A92A" "30553261" "3FA16872" "B020C49C" "3F50624D"
+    "D2F1A9FC" "3FB2F1A9" "FBE76C8B" "00000000" "00000040" "00000002" "00000030")
+GOLDEN_FIELD_HEADER = bytes.fromhex(
+    "00000008" "50726573" "73757265" "00000001" "00000000" "00000000"
+    "00000008" "56656C6F" "63697479" "00000003" "00000000" "00000000")
+
+
+def origin_cube(n=4) -> H.Geometry:
+    """An n^3 fluid cube at grid 0..n-1 in one block, walls on every face (the
+    positions of the reference's DummyDataSource)."""
+    xs, ys, zs = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    coords = np.stack([xs.ravel(), ys.ravel(), zs.ravel()], axis=1)
+    m = coords.shape[0]
+    lt = np.zeros((m, 18), np.uint8)
+    ld = np.full((m, 18), -1.0, np.float32)
+    for d in range(1, 19):
+        nb = coords + np.array([CX[d], CY[d], CZ[d]])
+        out = ((nb < 0) | (nb >= n)).any(axis=1)
+        lt[out, d - 1] = H.LINK_WALL
+        ld[out, d - 1] = 0.5
+    return H.Geometry((1, 1, 1), coords, lt, np.full((m, 18), -1, np.int32), ld,
+                      np.zeros(m, np.uint8), np.zeros((m, 3), np.float32), [], block_size=8)
+
+
+class Xtr:
+    """Independent .xtr parser (XDR big-endian)."""
+
+    def __init__(self, path):
+        b = open(path, "rb").read()
+        self.raw = b
+        (self.magic, self.xmagic, self.version, self.voxel, ox, oy, oz, self.sites, self.nfields,
+         self.fhlen) = struct.unpack(">IIId3dQII", b[:60])
+        self.origin = (ox, oy, oz)
+        at = 60
+        self.fields = []
+        while at < 60 + self.fhlen:
+            (ln,) = struct.unpack(">I", b[at:at + 4])
+            name = b[at + 4:at + 4 + ln].decode()
+            at += 4 + (ln + 3) // 4 * 4
+            nfl, off = struct.unpack(">Id", b[at:at + 12])
+            at += 12
+            self.fields.append((name, nfl, off))
+        assert at == 60 + self.fhlen
+        self.header_len = at
+        per_site = 12 + 4 * sum(f[1] for f in self.fields)
+        self.record_len = 8 + per_site * self.sites
+        body = b[at:]
+        assert len(body) % self.record_len == 0
+        self.records = []
+        for r in range(len(body) // self.record_len):
+            rec = body[r * self.record_len:(r + 1) * self.record_len]
+            (t,) = struct.unpack(">Q", rec[:8])
+            rows = np.frombuffer(rec[8:], dtype=">u4").reshape(self.sites, per_site // 4)
+            coords = rows[:, :3].astype(np.int64)
+            vals = rows[:, 3:].copy().view(">f4").astype(np.float32)
+            out, c = {}, 0
+            for name, nfl, _ in self.fields:
+                out[name] = vals[:, c:c + nfl]
+                c += nfl
+            self.records.append((int(t), coords, out))
+
+
+def lcg(seed):
+    """The LCG of the reference's test data (MakeDummyExtraction.py:51-80)."""
+    state = np.uint32(seed)
+    norm = np.float32(np.uint32(0xFFFFFFFF))
+    while True:
+        state = np.uint32((int(state) * 1664525 + 1013904223) & 0xFFFFFFFF)
+        yield np.float32(state) / norm
+
+
+def test_headers_match_reference_golden_bytes(tmp_path):
+    lat = H.Lattice(origin_cube(), None)
+    path = tmp_path / "dummy.xtr"
+    with H.Extraction(lat, path, [H.FIELD_PRESSURE, H.FIELD_VELOCITY], voxel_size=0.3e-3, time_step=1e-4,
+                      origin=(0.034, 0.001, 0.074), names=["Pressure", "Velocity"]) as x:
+        assert x.site_count == 64
+    b = open(path, "rb").read()
+    assert b[:60] == GOLDEN_MAIN_HEADER
+    assert b[60:108] == GOLDEN_FIELD_HEADER
+    assert len(b) == 108
+    assert not (tmp_path / "dummy.off").exists()  # only written with distributions
+
+
+def test_records_convert_units_like_the_reference(tmp_path):
+    """Physical pressure and velocity of the reference's test data survive the
+    round trip to lattice units and back through the file, within the
+    reference test's 1e-5 (LocalPropertyOutputTests.h:48, 222-237)."""
+    lat = H.Lattice(origin_cube(), None)
+    dx, dt = 0.3e-3, 1e-4
+    c = dx / dt  # lattice speed
+    path = tmp_path / "dummy.xtr"
+    rng = lcg(1358)
+    x = H.Extraction(lat, path, [H.FIELD_PRESSURE, H.FIELD_VELOCITY], voxel_size=dx, time_step=dt,
+                     origin=(0.034, 0.001, 0.074), names=["Pressure", "Velocity"])
+    want = []
+    for t in (0, 100):
+        # DummyDataSource::FillFields order: per site a pressure, then 3 velocities
+        p, v = np.zeros(64), np.zeros((64, 3))
+        for i in range(64):
+            p[i] = 80.0 + 2.0 * float(next(rng))
+            v[i] = [0.01 * float(next(rng)) for _ in range(3)]
+        # p = (rho - 1) Cs2 * rho_blood c^2 / MMHG; u_lattice = v / c
+        rho = 1.0 + p * MMHG / (1000.0 * c * c) / CS2
+        x.write_fields(t, rho=rho, u=v / c)
+        want.append((t, p, v))
+    x.close()
+    f = Xtr(path)
+    assert [r[0] for r in f.records] == [0, 100]
+    # sites in the lattice's own order (collision classes first, as
+    # LatticeData numbers them), not the dummy source's x-major order
+    order = lat.site_coords()
+    grid = np.stack(np.meshgrid(np.arange(4), np.arange(4), np.arange(4), indexing="ij"), -1).reshape(-1, 3)
+    assert sorted(map(tuple, order)) == sorted(map(tuple, grid))
+    for (t, coords, vals), (tw, p, v) in zip(f.records, want):
+        assert np.array_equal(coords, order)
+        np.testing.assert_allclose(vals["Pressure"][:, 0] + 0.0, p, atol=1e-5, rtol=0)
+        np.testing.assert_allclose(vals["Velocity"], v, atol=1e-5, rtol=0)
+
+
+def test_shear_stress_and_type_names(tmp_path):
+    """Shear stress is wss * rho_blood * (dx/dt)^2 Pa; sites with no wall
+    carry the NO_VALUE marker, which overflows to inf as a float."""
+    g = closed_box(3, 3, 3)
+    lat = H.Lattice(g, None)
+    n = g.n_sites
+    wss = np.linspace(1e-4, 1e-3, n)
+    wss[n // 2] = H.NO_VALUE
+    dx, dt = 1e-4, 2e-6
+    path = tmp_path / "wss.xtr"
+    with H.Extraction(lat, path, [H.FIELD_SHEAR_STRESS, H.FIELD_MPI_RANK], voxel_size=dx, time_step=dt) as x:
+        x.write_fields(7, wss=wss)
+    f = Xtr(path)
+    assert [fl[0] for fl in f.fields] == ["shearstress", "mpirank"]
+    assert [fl[1] for fl in f.fields] == [1, 1]
+    got = f.records[0][2]["shearstress"][:, 0]
+    fin = wss != H.NO_VALUE
+    want = (wss[fin] * (1000.0 * (dx / dt) ** 2)).astype(np.float32)
+    np.testing.assert_allclose(got[fin], want, rtol=2e-7)
+    assert np.isinf(got[~fin]).all()
+    assert (f.records[0][2]["mpirank"] == 0).all()
+
+
+def test_checkpoint_round_trip_and_offsets(tmp_path):
+    g = closed_box(4, 3, 5)
+    lat = H.Lattice(g, None)
+    n = g.n_sites
+    rng = np.random.default_rng(5)
+    fs = {t: rng.uniform(0.01, 0.4, size=(n, 19)) for t in (10, 20, 30)}
+    path = tmp_path / "cp.xtr"
+    with H.Extraction(lat, path, [H.FIELD_DISTRIBUTIONS]) as x:
+        for t, f in fs.items():
+            x.write_fields(t, f=f)
+    parsed = Xtr(path)
+    assert parsed.fields == [("distributions", 19, 0.0)]
+    assert parsed.header_len == 92
+    off = open(tmp_path / "cp.off", "rb").read()
+    magic, omagic, ver, nranks, o0, o1 = struct.unpack(">IIIIQQ", off)
+    assert (magic, omagic, ver, nranks) == (0x686C6221, 0x6F666604, 1, 1)
+    assert (o0, o1) == (92, 92 + 8 + n * (12 + 76))
+    for t, f in fs.items():
+        tt, got = lat.checkpoint_read(path, t)
+        assert tt == t
+        assert np.array_equal(got, f.astype(np.float32).astype(np.float64))
+    tt, got = lat.checkpoint_read(path)  # no target: the last record
+    assert tt == 30 and np.array_equal(got, fs[30].astype(np.float32).astype(np.float64))
+    with pytest.raises(H.HlbError, match="not found"):
+        lat.checkpoint_read(path, 25)
+    with pytest.raises(H.HlbError, match="not found"):
+        lat.checkpoint_read(path, 40)
+
+
+def test_checkpoint_binary_search_finds_every_record(tmp_path):
+    g = closed_box(2, 2, 2)
+    lat = H.Lattice(g, None)
+    times = [1, 2, 3, 5, 8, 13, 21, 34, 55]
+    path = tmp_path / "many.xtr"
+    with H.Extraction(lat, path, [H.FIELD_DISTRIBUTIONS]) as x:
+        for t in times:
+            x.write_fields(t, f=np.full((g.n_sites, 19), float(t)))
+    for t in times:
+        tt, f = lat.checkpoint_read(path, t)
+        assert tt == t and (f == t).all()
+    for t in (0, 4, 56):
+        with pytest.raises(H.HlbError, match="not found"):
+            lat.checkpoint_read(path, t)
+
+
+def test_checkpoint_rejects_foreign_files(tmp_path):
+    g = closed_box(3, 3, 3)
+    lat = H.Lattice(g, None)
+    n = g.n_sites
+    f = np.full((n, 19), 0.05)
+    # two fields
+    p2 = tmp_path / "two.xtr"
+    with H.Extraction(lat, p2, [H.FIELD_PRESSURE, H.FIELD_DISTRIBUTIONS]) as x:
+        x.write_fields(1, rho=np.ones(n), f=f)
+    with pytest.raises(H.HlbError, match="exactly one field"):
+        lat.checkpoint_read(p2)
+    # renamed field
+    p3 = tmp_path / "named.xtr"
+    with H.Extraction(lat, p3, [H.FIELD_DISTRIBUTIONS], names=["fdist"]) as x:
+        x.write_fields(1, f=f)
+    with pytest.raises(H.HlbError):
+        lat.checkpoint_read(p3)
+    # another lattice
+    p4 = tmp_path / "cp.xtr"
+    with H.Extraction(lat, p4, [H.FIELD_DISTRIBUTIONS]) as x:
+        x.write_fields(3, f=f)
+    other = H.Lattice(closed_box(3, 3, 4), None)
+    with pytest.raises(H.HlbError, match="sites"):
+        other.checkpoint_read(p4)
+    shifted = H.Lattice(closed_box(3, 3, 3, seed=9), None)  # same sites, same coords: accepted
+    assert shifted.checkpoint_read(p4)[0] == 3
+    # subset of the sites (a plane) is not a checkpoint of the whole lattice
+    p5 = tmp_path / "plane.xtr"
+    with H.Extraction(lat, p5, [H.FIELD_DISTRIBUTIONS], selector=H.SELECT_PLANE, point=(0, 0, 2),
+                      normal=(0, 0, 1)) as x:
+        x.write_fields(3, f=f)
+    with pytest.raises(H.HlbError):
+        lat.checkpoint_read(p5)
+    # truncated record
+    b = open(p4, "rb").read()
+    p6 = tmp_path / "cut.xtr"
+    open(p6, "wb").write(b[:-4])
+    open(tmp_path / "cut.off", "wb").write(open(tmp_path / "cp.off", "rb").read())
+    with pytest.raises(H.HlbError, match="integer number"):
+        lat.checkpoint_read(p6)
+    # missing offset file
+    p7 = tmp_path / "nooff.xtr"
+    open(p7, "wb").write(b)
+    with pytest.raises(H.HlbError, match="offset"):
+        lat.checkpoint_read(p7)
+
+
+def test_open_refuses_existing_file_and_bad_specs(tmp_path):
+    lat = H.Lattice(closed_box(2, 2, 2), None)
+    path = tmp_path / "a.xtr"
+    H.Extraction(lat, path, [H.FIELD_VELOCITY]).close()
+    with pytest.raises(H.HlbError, match="cannot create"):
+        H.Extraction(lat, path, [H.FIELD_VELOCITY])
+    with pytest.raises(H.HlbError, match="no fields"):
+        H.Extraction(lat, tmp_path / "b.xtr", [])
+    with pytest.raises(H.HlbError, match="unsupported field"):
+        H.Extraction(lat, tmp_path / "c.xtr", [17])
+    with pytest.raises(H.HlbError, match="normal"):
+        H.Extraction(lat, tmp_path / "d.xtr", [H.FIELD_PRESSURE], selector=H.SELECT_PLANE, normal=(0, 0, 0))
+    with pytest.raises(H.HlbError, match="extension"):
+        H.Extraction(lat, tmp_path / "noext", [H.FIELD_DISTRIBUTIONS])
+    x = H.Extraction(lat, tmp_path / "e.xtr", [H.FIELD_PRESSURE])
+    with pytest.raises(H.HlbError, match="pressure"):
+        x.write_fields(1, u=np.zeros((lat.site_coords().shape[0], 3)))
+    with pytest.raises(H.HlbError, match="no state captured"):
+        x.write()
+    x.close()
+
+
+def selected(lat, tmp_path, name, **kw):
+    path = tmp_path / f"{name}.xtr"
+    with H.Extraction(lat, path, [H.FIELD_MPI_RANK], **kw) as x:
+        x.write_fields(0)
+    return Xtr(path).records[0][1]
+
+
+def test_selectors(tmp_path):
+    g = four_cube(8)  # fluid 1..6 in each axis, inlet face z=1, outlet face z=6
+    lat = H.Lattice(g, None)
+    coords = lat.site_coords()
+    whole = selected(lat, tmp_path, "whole")
+    assert np.array_equal(whole, coords)
+    # walls on the x/y faces only
+    surf = selected(lat, tmp_path, "surf", selector=H.SELECT_SURFACE)
+    on_xy_face = (np.isin(coords[:, 0], (1, 6)) | np.isin(coords[:, 1], (1, 6)))
+    assert np.array_equal(surf, coords[on_xy_face])
+    inl = selected(lat, tmp_path, "inlet", selector=H.SELECT_INLET)
+    outl = selected(lat, tmp_path, "outlet", selector=H.SELECT_OUTLET)
+    assert np.array_equal(inl, coords[coords[:, 2] == 1])
+    assert np.array_equal(outl, coords[coords[:, 2] == 6])
+    # plane z = 3 (lattice units 0.5 mm, origin shifted): exactly one layer
+    dx, org = 0.5e-3, (0.01, -0.02, 0.003)
+    pz = selected(lat, tmp_path, "plane", selector=H.SELECT_PLANE, voxel_size=dx, origin=org,
+                  point=(0.0, 0.0, org[2] + 3 * dx + 0.1 * dx), normal=(0.0, 0.0, 2.0))
+    assert np.array_equal(pz, coords[coords[:, 2] == 3])
+    # with a radius: the disc around (3.5, 3.5) of radius 1.6 lattice units
+    centre = (org[0] + 3.5 * dx, org[1] + 3.5 * dx, org[2] + 3 * dx)
+    pr = selected(lat, tmp_path, "disc", selector=H.SELECT_PLANE, voxel_size=dx, origin=org, point=centre,
+                  normal=(0.0, 0.0, 1.0), radius=1.6 * dx)
+    r2 = (coords[:, 0] - 3.5) ** 2 + (coords[:, 1] - 3.5) ** 2
+    assert np.array_equal(pr, coords[(coords[:, 2] == 3) & (r2 <= 1.6 ** 2)])
+    # sphere of radius 2.2 around grid (3, 4, 3)
+    sp = selected(lat, tmp_path, "sphere", selector=H.SELECT_SPHERE, voxel_size=dx, origin=org,
+                  point=(org[0] + 3 * dx, org[1] + 4 * dx, org[2] + 3 * dx), radius=2.2 * dx)
+    d2 = ((coords - np.array([3, 4, 3])) ** 2).sum(axis=1)
+    assert np.array_equal(sp, coords[d2 <= 2.2 ** 2])
+    # an oblique plane through the cube's centre with normal (1, 1, 0): |x + y - 7| / sqrt2 <= 0.5
+    ob = selected(lat, tmp_path, "oblique", selector=H.SELECT_PLANE, point=(3.5, 3.5, 0.0), normal=(1.0, 1.0, 0.0))
+    dist = np.abs(coords[:, 0] + coords[:, 1] - 7.0) / np.sqrt(2.0)
+    assert np.array_equal(ob, coords[dist <= 0.5])

@@ -1,0 +1,127 @@
+"""Extraction and checkpoint restart on the GPU, against the oracle.
+
+A record written from the device (hlb_xtr_capture before step T, step T with
+properties, hlb_xtr_write) must be byte-identical to the record the same
+writer makes from the oracle's state entering step T and its step-T property
+caches, which is what the reference writes at the end of step T.  A restart
+from a checkpoint record must continue exactly like the oracle restarted from
+the same float-rounded distributions at the record's timestep.
+"""
+import numpy as np
+import pytest
+
+import hemepure_amd as H
+from fixtures import cylinder_case, four_cube, Case, IoletSpec
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = [H.FIELD_PRESSURE, H.FIELD_VELOCITY, H.FIELD_SHEAR_STRESS, H.FIELD_DISTRIBUTIONS]
+
+
+def pulsatile_cylinder():
+    return cylinder_case(radius=6.0, length=40, box_xy=16, rho_in=1.002, rho_out=0.999, amp=5e-4, period=50.0)
+
+
+@pytest.mark.parametrize("selector", [H.SELECT_WHOLE, H.SELECT_SURFACE, H.SELECT_PLANE])
+def test_record_matches_oracle_record(tmp_path, selector):
+    case = pulsatile_cylinder()
+    T = 37
+    lat = case.product()
+    ref = case.oracle()
+    lat.init_equilibrium(1.0, 1)
+    ref.init_equilibrium(1.0, 1)
+    kw = dict(voxel_size=5e-5, time_step=1e-5, origin=(0.001, 0.002, -0.003), selector=selector,
+              point=(0.001 + 8 * 5e-5, 0.0, 0.0), normal=(1.0, 0.0, 0.0))
+    dev = H.Extraction(lat, tmp_path / "dev.xtr", FIELDS, **kw)
+    want = H.Extraction(lat, tmp_path / "want.xtr", FIELDS, **kw)
+    assert dev.site_count > 0
+    for t_rec in (T, T + 5):
+        lat.step(t_rec - lat.timestep())
+        ref.run(t_rec - ref.timestep(), threads=0)
+        assert lat.timestep() == ref.timestep() == t_rec
+        dev.capture()
+        f_in = ref.f()
+        lat.step(1, properties=True)
+        ref.run(1, properties=True, threads=0)
+        assert dev.write() == t_rec
+        rho, u, wss = ref.fields()
+        want.write_fields(t_rec, rho=rho, u=u, wss=wss, f=f_in)
+    dev.close()
+    want.close()
+    a = open(tmp_path / "dev.xtr", "rb").read()
+    b = open(tmp_path / "want.xtr", "rb").read()
+    assert len(a) == len(b)
+    assert a == b
+    if selector == H.SELECT_WHOLE:
+        assert open(tmp_path / "dev.off", "rb").read() == open(tmp_path / "want.off", "rb").read()
+
+
+def test_write_needs_exactly_the_captured_step(tmp_path):
+    case = pulsatile_cylinder()
+    lat = case.product()
+    lat.init_equilibrium(1.0, 1)
+    x = H.Extraction(lat, tmp_path / "a.xtr", [H.FIELD_VELOCITY])
+    x.capture()
+    lat.step(2, properties=True)
+    with pytest.raises(H.HlbError, match="captured step"):
+        x.write()
+    x.capture()
+    lat.step(1, properties=True)
+    assert x.write() == 3
+    x.close()
+
+
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32_SHIFTED])
+def test_checkpoint_restart_matches_oracle(tmp_path, precision):
+    case = pulsatile_cylinder()
+    lat = case.product(precision)
+    lat.init_equilibrium(1.0, 1)
+    lat.step(40)
+    path = tmp_path / "cp.xtr"
+    with H.Extraction(lat, path, [H.FIELD_DISTRIBUTIONS]) as x:
+        x.capture()
+        assert x.write() == 41
+        lat.step(9)
+        x.capture()
+        assert x.write() == 50
+    fresh = case.product()
+    t, f = fresh.checkpoint_read(path, 41)
+    assert t == 41
+    t_last, f_last = fresh.checkpoint_read(path)
+    assert t_last == 50
+    assert np.array_equal(f_last, lat.download_f().astype(np.float32).astype(np.float64))
+    fresh.upload_f(f, t)
+    ref = case.oracle()
+    ref.set_f(f, t)
+    fresh.step(30, properties=True)
+    ref.run(30, properties=True, threads=0)
+    assert fresh.timestep() == ref.timestep() == 71
+    assert np.array_equal(fresh.download_f(), ref.f())
+    rho, u, _ = fresh.fields()
+    orho, ou, _ = ref.fields()
+    assert np.array_equal(rho, orho) and np.array_equal(u, ou)
+
+
+def test_inlet_outlet_selectors_on_device_lattice(tmp_path):
+    g = four_cube(8)
+    i0, o0 = g.inlets()[0], g.outlets()[0]
+    case = Case(g, inlets=[IoletSpec(H.COSINE, i0["position"], i0["normal"], density_mean=1.001)],
+                outlets=[IoletSpec(H.COSINE, o0["position"], o0["normal"], density_mean=0.999)])
+    lat = case.product()
+    ref = case.oracle()
+    lat.init_equilibrium(1.0, 1)
+    ref.init_equilibrium(1.0, 1)
+    lat.step(10)
+    ref.run(10, threads=0)
+    for sel in (H.SELECT_INLET, H.SELECT_OUTLET):
+        dev = H.Extraction(lat, tmp_path / f"d{sel}.xtr", [H.FIELD_PRESSURE, H.FIELD_DISTRIBUTIONS], selector=sel)
+        want = H.Extraction(lat, tmp_path / f"w{sel}.xtr", [H.FIELD_PRESSURE, H.FIELD_DISTRIBUTIONS], selector=sel)
+        dev.capture()
+        f_in = ref.f()
+        lat.step(1, properties=True)
+        ref.run(1, properties=True, threads=0)
+        dev.write()
+        want.write_fields(lat.timestep() - 1, rho=ref.fields()[0], f=f_in)
+        dev.close()
+        want.close()
+        assert open(tmp_path / f"d{sel}.xtr", "rb").read() == open(tmp_path / f"w{sel}.xtr", "rb").read()
