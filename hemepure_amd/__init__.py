@@ -23,8 +23,11 @@ IOLET_NASH, IOLET_LADD = 0, 1
 COSINE, PARABOLIC, FILEVEL, WK2, WK3 = 0, 1, 2, 3, 4
 FP64, FP32_SHIFTED = 0, 1
 STEP_PROPERTIES = 1
-FIELD_PRESSURE, FIELD_VELOCITY, FIELD_SHEAR_STRESS, FIELD_DISTRIBUTIONS, FIELD_MPI_RANK = 0, 1, 2, 3, 4
-SELECT_WHOLE, SELECT_SURFACE, SELECT_INLET, SELECT_OUTLET, SELECT_PLANE, SELECT_SPHERE = 0, 1, 2, 3, 4, 5
+(FIELD_PRESSURE, FIELD_VELOCITY, FIELD_SHEAR_STRESS, FIELD_VON_MISES_STRESS, FIELD_SHEAR_RATE,
+ FIELD_STRESS_TENSOR, FIELD_TRACTION, FIELD_TANGENTIAL_PROJECTION_TRACTION, FIELD_NORMAL_PROJECTION_TRACTION,
+ FIELD_WALL_EXTENSION, FIELD_DISTRIBUTIONS, FIELD_MPI_RANK) = range(12)
+(SELECT_WHOLE, SELECT_SURFACE, SELECT_INLET, SELECT_OUTLET, SELECT_PLANE, SELECT_SPHERE, SELECT_LINE,
+ SELECT_SURFACE_POINT, SELECT_SURFACE_SPHERE) = range(9)
 Q = 19
 NO_VALUE = np.finfo(np.float64).max
 
@@ -129,7 +132,8 @@ class XtrSpec(C.Structure):
     _fields_ = [("fields", C.POINTER(C.c_int32)), ("names", C.POINTER(C.c_char_p)), ("n_fields", C.c_int32),
                 ("selector", C.c_int32),
                 ("voxel_size", C.c_double), ("time_step", C.c_double), ("origin", C.c_double * 3),
-                ("point", C.c_double * 3), ("normal", C.c_double * 3), ("radius", C.c_double)]
+                ("point", C.c_double * 3), ("normal", C.c_double * 3), ("radius", C.c_double),
+                ("point2", C.c_double * 3)]
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_CONVERGED, REQ_DONE, REQ_FAILED = 0, 1, 2, 3, 4
@@ -184,7 +188,8 @@ _SIGS = {
     "hlb_xtr_capture": (C.c_int, [C.c_void_p]),
     "hlb_xtr_write": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "hlb_xtr_write_fields": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
-                                       C.c_void_p]),
+                                       C.c_void_p, C.c_void_p]),
+    "hlb_get_stress": (C.c_int, [C.c_void_p] + [C.c_void_p] * 6),
     "hlb_xtr_site_count": (C.c_int64, [C.c_void_p]),
     "hlb_xtr_close": (None, [C.c_void_p]),
     "hlb_checkpoint_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.POINTER(C.c_int64),
@@ -481,6 +486,16 @@ class Lattice:
         _check(lib().hlb_get_fields(self._h, _ptr(rho), _ptr(u), _ptr(wss)))
         return rho, u, wss
 
+    def stress(self) -> np.ndarray:
+        """Stress caches from the state entering the next step, n x 17: stress
+        tensor (xx, xy, xz, yy, yz, zz), von Mises, shear rate, traction,
+        tangential and normal projections of the traction."""
+        n = self.n
+        ten, vm, rate = np.empty((n, 6)), np.empty(n), np.empty(n)
+        tr, tt, tn = np.empty((n, 3)), np.empty((n, 3)), np.empty((n, 3))
+        _check(lib().hlb_get_stress(self._h, _ptr(ten), _ptr(vm), _ptr(rate), _ptr(tr), _ptr(tt), _ptr(tn)))
+        return np.concatenate([ten, vm[:, None], rate[:, None], tr, tt, tn], axis=1)
+
     def timestep(self) -> int:
         return int(lib().hlb_timestep(self._h))
 
@@ -528,7 +543,8 @@ class Extraction:
     """A property-output file (.xtr, plus .off with distributions) of one lattice."""
 
     def __init__(self, lat: "Lattice", path, fields, selector=SELECT_WHOLE, voxel_size=1.0, time_step=1.0,
-                 origin=(0.0, 0.0, 0.0), point=(0.0, 0.0, 0.0), normal=(0.0, 0.0, 1.0), radius=0.0, names=None):
+                 origin=(0.0, 0.0, 0.0), point=(0.0, 0.0, 0.0), normal=(0.0, 0.0, 1.0), radius=0.0, names=None,
+                 point2=(0.0, 0.0, 0.0)):
         self._lat = lat
         self._fields = (C.c_int32 * len(fields))(*fields)
         self._names = None
@@ -537,7 +553,8 @@ class Extraction:
         spec = XtrSpec(C.cast(self._fields, C.POINTER(C.c_int32)),
                        None if self._names is None else C.cast(self._names, C.POINTER(C.c_char_p)),
                        len(fields), selector, voxel_size, time_step,
-                       (C.c_double * 3)(*origin), (C.c_double * 3)(*point), (C.c_double * 3)(*normal), radius)
+                       (C.c_double * 3)(*origin), (C.c_double * 3)(*point), (C.c_double * 3)(*normal), radius,
+                       (C.c_double * 3)(*point2))
         h = C.c_void_p()
         _check(lib().hlb_xtr_open(lat._h, str(path).encode(), C.byref(spec), C.byref(h)))
         self._h = h
@@ -571,7 +588,7 @@ class Extraction:
         _check(lib().hlb_xtr_write(self._h, C.byref(t)))
         return int(t.value)
 
-    def write_fields(self, timestep: int, rho=None, u=None, wss=None, f=None):
+    def write_fields(self, timestep: int, rho=None, u=None, wss=None, f=None, stress=None):
         def p(a, shape):
             if a is None:
                 return None
@@ -579,7 +596,7 @@ class Extraction:
             assert a.size == int(np.prod(shape))
             return a
         n = int(lib().hlb_site_count(self._lat._h))
-        arrs = [p(rho, (n,)), p(u, (n, 3)), p(wss, (n,)), p(f, (n, Q))]
+        arrs = [p(rho, (n,)), p(u, (n, 3)), p(wss, (n,)), p(f, (n, Q)), p(stress, (n, 17))]
         _check(lib().hlb_xtr_write_fields(self._h, int(timestep), *[None if a is None else _ptr(a) for a in arrs]))
 
 

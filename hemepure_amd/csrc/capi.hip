@@ -1129,6 +1129,36 @@ extern "C" int hlb_get_fields(hlb_handle h, double* rho, double* u, double* wss)
   return HLB_OK;
 }
 
+extern "C" int hlb_get_stress(hlb_handle h, double* tensor, double* von_mises, double* shear_rate,
+                              double* traction, double* tangential, double* normal) {
+  if (!h) return fail(HLB_EINVAL, "hlb_get_stress: null handle");
+  if (!h->f_valid) return fail(HLB_EINVAL, "hlb_get_stress: no state");
+  int rc = materialise_f(h);
+  if (rc != HLB_OK) return rc;
+  const int64_t N = h->L.N, S = h->S;
+  if (!h->stress.p) HLB_CHECK(h->stress.alloc((size_t)17 * S));
+  const StepArgs a = make_args(h, nullptr, nullptr);
+  if (h->fp32()) HLB_CHECK(launch_stress_fp64(a, h->params.kernel, h->F32.p, true, h->stress.p, h->stream));
+  else HLB_CHECK(launch_stress_fp64(a, h->params.kernel, h->F.p, false, h->stress.p, h->stream));
+  HLB_CHECK(hipStreamSynchronize(h->stream));
+  std::vector<double> tmp((size_t)S);
+  // (destination, first plane, planes)
+  const struct {
+    double* dst;
+    int first, n;
+  } parts[] = {{tensor, 0, 6}, {von_mises, 6, 1}, {shear_rate, 7, 1}, {traction, 8, 3}, {tangential, 11, 3},
+               {normal, 14, 3}};
+  for (const auto& p : parts) {
+    if (!p.dst) continue;
+    for (int c = 0; c < p.n; ++c) {
+      HLB_CHECK(hipMemcpy(tmp.data(), h->stress.p + (size_t)(p.first + c) * S, sizeof(double) * S,
+                          hipMemcpyDeviceToHost));
+      for (int64_t k = 0; k < N; ++k) p.dst[(size_t)k * p.n + c] = tmp[h->inv[k]];
+    }
+  }
+  return HLB_OK;
+}
+
 extern "C" int64_t hlb_timestep(hlb_handle h) { return h ? h->t : -1; }
 
 extern "C" int hlb_stability(hlb_handle h, int* stable) {

@@ -111,28 +111,53 @@ struct XdrIn {
   }
 };
 
-uint32_t field_floats(int32_t f) {
+uint32_t field_floats(int32_t f) {  // LocalPropertyOutput::GetFieldLength (:359-393)
   switch (f) {
     case HLB_FIELD_PRESSURE:
+    case HLB_FIELD_VON_MISES_STRESS:
     case HLB_FIELD_SHEAR_STRESS:
+    case HLB_FIELD_SHEAR_RATE:
+    case HLB_FIELD_WALL_EXTENSION:
     case HLB_FIELD_MPI_RANK: return 1;
-    case HLB_FIELD_VELOCITY: return 3;
+    case HLB_FIELD_VELOCITY:
+    case HLB_FIELD_TRACTION:
+    case HLB_FIELD_TANGENTIAL_PROJECTION_TRACTION:
+    case HLB_FIELD_NORMAL_PROJECTION_TRACTION: return 3;
+    case HLB_FIELD_STRESS_TENSOR: return 6;
     case HLB_FIELD_DISTRIBUTIONS: return 19;
     default: return 0;
   }
 }
 
+// The type names of SimConfig::DoIOForPropertyField (SimConfig.cc:581-630).
 const char* field_name(int32_t f) {
   switch (f) {
     case HLB_FIELD_PRESSURE: return "pressure";
     case HLB_FIELD_VELOCITY: return "velocity";
     case HLB_FIELD_SHEAR_STRESS: return "shearstress";
+    case HLB_FIELD_VON_MISES_STRESS: return "vonmisesstress";
+    case HLB_FIELD_SHEAR_RATE: return "shearrate";
+    case HLB_FIELD_STRESS_TENSOR: return "stresstensor";
+    case HLB_FIELD_TRACTION: return "traction";
+    case HLB_FIELD_TANGENTIAL_PROJECTION_TRACTION: return "tangentialprojectiontraction";
+    case HLB_FIELD_NORMAL_PROJECTION_TRACTION: return "normalprojectiontraction";
+    case HLB_FIELD_WALL_EXTENSION: return "wallextension";
     case HLB_FIELD_DISTRIBUTIONS: return "distributions";
     default: return "mpirank";
   }
 }
 
-// LocalPropertyOutput::GetOffset (LocalPropertyOutput.cc:395-403).
+// Fields computed from the step's property caches (density, velocity, WSS)
+// and from the stress caches.
+bool needs_fields(int32_t f) {
+  return f == HLB_FIELD_PRESSURE || f == HLB_FIELD_VELOCITY || f == HLB_FIELD_SHEAR_STRESS;
+}
+bool needs_stress(int32_t f) {
+  return f == HLB_FIELD_VON_MISES_STRESS || f == HLB_FIELD_SHEAR_RATE || f == HLB_FIELD_STRESS_TENSOR ||
+         f == HLB_FIELD_TRACTION || f == HLB_FIELD_TANGENTIAL_PROJECTION_TRACTION ||
+         f == HLB_FIELD_NORMAL_PROJECTION_TRACTION;
+}
+
 double field_offset(int32_t f) { return f == HLB_FIELD_PRESSURE ? kReferencePressure : 0.0; }
 
 struct Vec3f {
@@ -141,6 +166,10 @@ struct Vec3f {
 inline float dot(Vec3f a, Vec3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 inline Vec3f sub(Vec3f a, Vec3f b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 inline Vec3f scale(Vec3f a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline Vec3f scale_div(Vec3f a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+inline Vec3f add(Vec3f a, Vec3f b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+// SurfacePointSelector::maxDistanceBetweenSitesLatticeUnits (SurfacePointSelector.cc:17).
+const float kMaxSurfaceDistance = (float)std::sqrt(3.0);
 // Vector3D<float>(location) * voxelSize + origin, evaluated in double and
 // stored as float (Vector3DArithmeticTraits<float, double> is double).
 inline Vec3f physical(const int64_t* g, double dx, const double* origin) {
@@ -173,10 +202,12 @@ struct hlb_xtr {
   std::vector<std::string> names;
   std::vector<int64_t> sites;  // selected sites, lattice order
   std::vector<int64_t> coords;
-  double latticePressure = 0.0, latticeSpeed = 0.0;
-  bool wantFields = false, wantF = false;
-  int64_t captured = -1;  // step whose entering state f holds
-  std::vector<double> f;
+  double latticePressure = 0.0, latticeSpeed = 0.0, latticeTime = 0.0, voxelSize = 0.0;
+  bool wantFields = false, wantF = false, wantStress = false;
+  int64_t captured = -1;   // step whose entering state f and stress hold
+  std::vector<double> f;       // n*19
+  std::vector<double> stress;  // n*17, the hlb_get_stress layout
+  std::vector<double> normal;  // n*3, GetWallNormal (inf where none)
 };
 
 extern "C" int hlb_xtr_open(hlb_handle h, const char* path, const hlb_xtr_spec* spec, hlb_xtr_handle* out) {
@@ -193,8 +224,8 @@ extern "C" int hlb_xtr_open(hlb_handle h, const char* path, const hlb_xtr_spec* 
     for (int i = 0; i < spec->n_fields; ++i) {
       if (field_floats(spec->fields[i]) == 0) return fail(HLB_EINVAL, "hlb_xtr_open: unsupported field");
       dist = dist || spec->fields[i] == HLB_FIELD_DISTRIBUTIONS;
-      x->wantFields = x->wantFields || spec->fields[i] == HLB_FIELD_PRESSURE ||
-                      spec->fields[i] == HLB_FIELD_VELOCITY || spec->fields[i] == HLB_FIELD_SHEAR_STRESS;
+      x->wantFields = x->wantFields || needs_fields(spec->fields[i]);
+      x->wantStress = x->wantStress || needs_stress(spec->fields[i]);
       x->fields.push_back(spec->fields[i]);
       const char* nm = spec->names ? spec->names[i] : nullptr;
       x->names.emplace_back(nm ? nm : field_name(spec->fields[i]));
@@ -203,6 +234,8 @@ extern "C" int hlb_xtr_open(hlb_handle h, const char* path, const hlb_xtr_spec* 
     const double dx = spec->voxel_size, dt = spec->time_step;
     x->latticePressure = kBloodDensity * dx * dx * dx / (dx * dt * dt);
     x->latticeSpeed = dx / dt;
+    x->latticeTime = dt;
+    x->voxelSize = dx;
     const int64_t N = hlb_site_count(h);
     x->coords.resize((size_t)N * 3);
     int rc = hlb_site_coords(h, x->coords.data());
@@ -210,6 +243,9 @@ extern "C" int hlb_xtr_open(hlb_handle h, const char* path, const hlb_xtr_spec* 
     std::vector<uint32_t> wall(N), iol(N);
     std::vector<int32_t> id(N), type(N);
     rc = hlb_site_masks(h, wall.data(), iol.data(), id.data(), type.data());
+    if (rc != HLB_OK) return rc;
+    x->normal.resize((size_t)N * 3);
+    rc = hlb_wall_geometry(h, nullptr, x->normal.data());
     if (rc != HLB_OK) return rc;
     // Selector geometry in float, as the selectors hold it
     // (PlaneGeometrySelector.h:30-60, SphereGeometrySelector.h:29-61).
@@ -221,6 +257,13 @@ extern "C" int hlb_xtr_open(hlb_handle h, const char* path, const hlb_xtr_spec* 
       normal = {normal.x / mag, normal.y / mag, normal.z / mag};
     }
     const float radius = (float)spec->radius;
+    // StraightLineGeometrySelector's members: the vector from end 1 to end 2
+    // and its length, in float.
+    const Vec3f point2{(float)spec->point2[0], (float)spec->point2[1], (float)spec->point2[2]};
+    const Vec3f lineVector = sub(point2, point);
+    const float lineLength = std::sqrt(dot(lineVector, lineVector));
+    if (spec->selector == HLB_SELECT_LINE && !(lineLength > 0.0f))
+      return fail(HLB_EINVAL, "hlb_xtr_open: the line has zero length");
     for (int64_t k = 0; k < N; ++k) {
       bool in = true;
       switch (spec->selector) {
@@ -239,9 +282,37 @@ extern "C" int hlb_xtr_open(hlb_handle h, const char* path, const hlb_xtr_spec* 
           }
           break;
         }
+        case HLB_SELECT_SURFACE_SPHERE:                     // SurfaceWithinSphereSelector.cc:18-22
+          if (wall[k] == 0) {
+            in = false;
+            break;
+          }
+          [[fallthrough]];
         case HLB_SELECT_SPHERE: {                           // SphereGeometrySelector.cc:30-36
           const Vec3f e = sub(physical(&x->coords[3 * k], dx, spec->origin), point);
           in = std::sqrt(dot(e, e)) <= radius;
+          break;
+        }
+        case HLB_SELECT_LINE: {                             // StraightLineGeometrySelector.cc:33-57
+          const Vec3f c = physical(&x->coords[3 * k], dx, spec->origin);
+          const float along = dot(lineVector, sub(c, point)) / lineLength;
+          if (along < 0. || along > lineLength) {
+            in = false;
+            break;
+          }
+          const Vec3f foot = add(point, scale_div(scale(lineVector, along), lineLength));
+          const Vec3f e = sub(foot, c);
+          in = dot(e, e) <= (2.0 * 0.5 * 0.5 * dx * dx);
+          break;
+        }
+        case HLB_SELECT_SURFACE_POINT: {                    // SurfacePointSelector.cc:24-40
+          if (wall[k] == 0) {
+            in = false;
+            break;
+          }
+          const Vec3f e = sub(physical(&x->coords[3 * k], dx, spec->origin), point);
+          const float dist = std::sqrt(dot(e, e)) / dx;
+          in = dist <= kMaxSurfaceDistance;
           break;
         }
         default: return fail(HLB_EINVAL, "hlb_xtr_open: unknown selector");
@@ -292,18 +363,22 @@ extern "C" int hlb_xtr_open(hlb_handle h, const char* path, const hlb_xtr_spec* 
 }
 
 extern "C" int hlb_xtr_write_fields(hlb_xtr_handle x, int64_t timestep, const double* rho, const double* u,
-                                    const double* wss, const double* f) {
+                                    const double* wss, const double* f, const double* stress) {
   if (!x) return fail(HLB_EINVAL, "hlb_xtr_write_fields: null handle");
   for (int32_t fl : x->fields) {
     const bool missing = (fl == HLB_FIELD_PRESSURE && !rho) || (fl == HLB_FIELD_VELOCITY && !u) ||
-                         (fl == HLB_FIELD_SHEAR_STRESS && !wss) || (fl == HLB_FIELD_DISTRIBUTIONS && !f);
+                         (fl == HLB_FIELD_SHEAR_STRESS && !wss) || (fl == HLB_FIELD_DISTRIBUTIONS && !f) ||
+                         (needs_stress(fl) && !stress);
     if (missing) return fail(HLB_EINVAL, std::string("hlb_xtr_write_fields: no data for field ") + field_name(fl));
   }
   try {
     Xdr w;
     w.u64((uint64_t)timestep);
+    // LocalPropertyOutput::Write (:216-327) with LbDataSourceIterator's
+    // conversions (LbDataSourceIterator.cc:39-88, UnitConverter.h:60-100).
     for (int64_t k : x->sites) {
       for (int a = 0; a < 3; ++a) w.u32((uint32_t)x->coords[3 * k + a]);
+      const double* st = stress ? stress + 17 * k : nullptr;
       for (int32_t fl : x->fields) {
         switch (fl) {
           case HLB_FIELD_PRESSURE: {
@@ -316,6 +391,34 @@ extern "C" int hlb_xtr_write_fields(hlb_xtr_handle x, int64_t timestep, const do
             for (int a = 0; a < 3; ++a) w.f32((float)(u[3 * k + a] * x->latticeSpeed));
             break;
           case HLB_FIELD_SHEAR_STRESS: w.f32((float)(wss[k] * x->latticePressure)); break;
+          case HLB_FIELD_VON_MISES_STRESS: w.f32((float)(st[6] * x->latticePressure)); break;
+          case HLB_FIELD_SHEAR_RATE: w.f32((float)(st[7] / x->latticeTime)); break;
+          case HLB_FIELD_STRESS_TENSOR:
+            // stressTensor * latticePressure, diagonal shifted by -REFERENCE_PRESSURE_mmHg * mmHg_TO_PASCAL
+            for (int c = 0; c < 6; ++c) {
+              double v = x->latticePressure * st[c];
+              if (c == 0 || c == 3 || c == 5) v += -kReferencePressure * kMmHgToPa;
+              w.f32((float)v);
+            }
+            break;
+          case HLB_FIELD_TRACTION:
+            // ConvertTractionToPhysicalUnits: traction * (speed^2 rho_blood) - normal * REF * mmHg_TO_PASCAL
+            for (int a = 0; a < 3; ++a) {
+              double v = st[8 + a] * (x->latticeSpeed * x->latticeSpeed * kBloodDensity);
+              v -= x->normal[3 * k + a] * kReferencePressure * kMmHgToPa;
+              w.f32((float)v);
+            }
+            break;
+          case HLB_FIELD_TANGENTIAL_PROJECTION_TRACTION:
+            for (int a = 0; a < 3; ++a) w.f32((float)(st[11 + a] * x->latticePressure));
+            break;
+          case HLB_FIELD_NORMAL_PROJECTION_TRACTION:
+            for (int a = 0; a < 3; ++a) w.f32((float)(st[14 + a] * x->latticePressure));
+            break;
+          case HLB_FIELD_WALL_EXTENSION:
+            // set only by the elastic-wall delegate, which this library does not have
+            w.f32((float)(0.0 * x->voxelSize));
+            break;
           case HLB_FIELD_DISTRIBUTIONS:
             for (int d = 0; d < 19; ++d) w.f32((float)f[k * 19 + d]);
             break;
@@ -338,16 +441,36 @@ extern "C" int hlb_xtr_write_fields(hlb_xtr_handle x, int64_t timestep, const do
 // ran with properties, writes record T.
 extern "C" int hlb_xtr_capture(hlb_xtr_handle x) {
   if (!x) return fail(HLB_EINVAL, "hlb_xtr_capture: null handle");
-  x->captured = hlb_timestep(x->lattice);
-  if (!x->wantF) return HLB_OK;
+  x->captured = -1;
+  const size_t N = (size_t)hlb_site_count(x->lattice);
   try {
-    x->f.resize((size_t)hlb_site_count(x->lattice) * 19);
+    if (x->wantF) {
+      x->f.resize(N * 19);
+      const int rc = hlb_download_f(x->lattice, x->f.data());
+      if (rc != HLB_OK) return rc;
+    }
+    if (x->wantStress) {
+      std::vector<double> ten(N * 6), vm(N), rate(N), tr(N * 3), tt(N * 3), tn(N * 3);
+      const int rc = hlb_get_stress(x->lattice, ten.data(), vm.data(), rate.data(), tr.data(), tt.data(), tn.data());
+      if (rc != HLB_OK) return rc;
+      x->stress.resize(N * 17);
+      for (size_t k = 0; k < N; ++k) {
+        double* o = &x->stress[17 * k];
+        for (int c = 0; c < 6; ++c) o[c] = ten[6 * k + c];
+        o[6] = vm[k];
+        o[7] = rate[k];
+        for (int a = 0; a < 3; ++a) {
+          o[8 + a] = tr[3 * k + a];
+          o[11 + a] = tt[3 * k + a];
+          o[14 + a] = tn[3 * k + a];
+        }
+      }
+    }
   } catch (const std::bad_alloc&) {
     return fail(HLB_EINVAL, "hlb_xtr_capture: out of host memory");
   }
-  const int rc = hlb_download_f(x->lattice, x->f.data());
-  if (rc != HLB_OK) x->captured = -1;
-  return rc;
+  x->captured = hlb_timestep(x->lattice);
+  return HLB_OK;
 }
 
 extern "C" int hlb_xtr_write(hlb_xtr_handle x, int64_t* timestep) {
@@ -367,7 +490,8 @@ extern "C" int hlb_xtr_write(hlb_xtr_handle x, int64_t* timestep) {
       if (rc != HLB_OK) return rc;
     }
     const int rc = hlb_xtr_write_fields(x, T, x->wantFields ? rho.data() : nullptr, x->wantFields ? u.data() : nullptr,
-                                        x->wantFields ? wss.data() : nullptr, x->wantF ? x->f.data() : nullptr);
+                                        x->wantFields ? wss.data() : nullptr, x->wantF ? x->f.data() : nullptr,
+                                        x->wantStress ? x->stress.data() : nullptr);
     if (rc != HLB_OK) return rc;
   } catch (const std::bad_alloc&) {
     return fail(HLB_EINVAL, "hlb_xtr_write: out of host memory");

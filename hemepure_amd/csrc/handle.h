@@ -25,6 +25,7 @@ size_t monitor_scratch(int64_t nslab);
 hipError_t launch_monitor(const BrickTables& bt, const double* rho, const double* u, int64_t S, void* scratch,
                           double* out, hipStream_t s);
 hipError_t launch_gather_fp64(const StepArgs& a, double* F, int64_t S, hipStream_t s);
+hipError_t launch_stress_fp64(const StepArgs& a, int kernel, const void* F, bool fp32, double* out, hipStream_t s);
 hipError_t launch_fill_fp64(double* F, int64_t N, int64_t S, const double* vals, hipStream_t s);
 hipError_t launch_unstable_fp64(const BrickTables& bt, const double* F, int64_t S, int* flag, hipStream_t s);
 hipError_t launch_wk_begin(const WkRef* refs, int n, int step, hipStream_t s);
@@ -125,6 +126,7 @@ struct hlb_lattice {
   hlb::DevBuf<double> P64, GW64;
   hlb::DevBuf<float> P32, GW32;
   hlb::DevBuf<char> mon_scratch;  // monitor partials, allocated on first use
+  hlb::DevBuf<double> stress;     // 17 stress planes, allocated on first use
   hlb::DevBuf<double> mon_out;
   // FP32-shifted storage (params.precision == HLB_FP32_SHIFTED): float A, F,
   // G and the one-member table the member-table kernel reads.

@@ -174,6 +174,52 @@ __device__ __forceinline__ double wall_shear_stress(const StepArgs& a, uint32_t 
   return __builtin_sqrt(sq - nn * nn);
 }
 
+// Collision of one site from its moments: LBGK (LBGK.h:64-72), TRT
+// (TRT.h:80-109) or LBGK with the LES relaxation time
+// (LBGKLESSpongeLayer.h:150-182, lifetime 0); tau returns the site's.
+template <int KERNEL>
+__device__ __forceinline__ void collide_site(const StepArgs& a, const double* f, double rho, double mx, double my,
+                                             double mz, double& tau, double* fpost) {
+  const FeqPre pre = feq_pre(rho, mx, my, mz);
+  double feq[Q];
+  feq_all(pre, mx, my, mz, feq);
+  double fneq[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) fneq[j] = f[j] - feq[j];
+  tau = a.tau;
+  if constexpr (KERNEL == HLB_KERNEL_LBGK) {
+#pragma unroll
+    for (int j = 0; j < Q; ++j) fpost[j] = f[j] + fneq[j] * a.omega;
+  } else if constexpr (KERNEL == HLB_KERNEL_TRT) {
+    // the (0,0) pair reduces to f0 + omega fneq0
+    fpost[0] = f[0] + a.omega * fneq[0];
+#pragma unroll
+    for (int i = 1; i < Q; i += 2) {
+      const int ib = i + 1;
+      const double sym = a.half_omega * (fneq[i] + fneq[ib]);
+      const double asym = a.half_omega_minus * (fneq[i] - fneq[ib]);
+      fpost[i] = f[i] + sym + asym;
+      fpost[ib] = f[ib] + sym - asym;
+    }
+  } else {
+    const Sym3 s = second_moments(fneq);
+    double q12 = s.xx * s.xx;
+    q12 += s.yx * s.yx;
+    q12 += s.zx * s.zx;
+    q12 += s.yx * s.yx;
+    q12 += s.yy * s.yy;
+    q12 += s.zy * s.zy;
+    q12 += s.zx * s.zx;
+    q12 += s.zy * s.zy;
+    q12 += s.zz * s.zz;
+    q12 = __builtin_sqrt(q12);
+    tau = 1. / 2. * (a.tau + __builtin_sqrt(a.tau * a.tau + a.les_k * q12));
+    const double om = -1.0 / tau;
+#pragma unroll
+    for (int j = 0; j < Q; ++j) fpost[j] = f[j] + fneq[j] * om;
+  }
+}
+
 // The slab record of this lane's wavefront: four slabs per workgroup.
 __device__ __forceinline__ uint32_t slab_record() {
   return xcd_tile() * (kBlock / kSlab) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -215,47 +261,9 @@ __global__ __launch_bounds__(B) void step_fp64(StepArgs a) {
   double rho, mx, my, mz;
   moments(f, rho, mx, my, mz);
   const double ux = mx / rho, uy = my / rho, uz = mz / rho;
-  const FeqPre pre = feq_pre(rho, mx, my, mz);
-  double feq[Q];
-  feq_all(pre, mx, my, mz, feq);
-  double fneq[Q];
-#pragma unroll
-  for (int j = 0; j < Q; ++j) fneq[j] = f[j] - feq[j];
-  double tau = a.tau;
+  double tau;
   double fpost[Q];
-  if constexpr (KERNEL == HLB_KERNEL_LBGK) {
-    // LBGK.h:64-72
-#pragma unroll
-    for (int j = 0; j < Q; ++j) fpost[j] = f[j] + fneq[j] * a.omega;
-  } else if constexpr (KERNEL == HLB_KERNEL_TRT) {
-    // TRT.h:80-109; the (0,0) pair reduces to f0 + omega fneq0.
-    fpost[0] = f[0] + a.omega * fneq[0];
-#pragma unroll
-    for (int i = 1; i < Q; i += 2) {
-      const int ib = i + 1;
-      const double sym = a.half_omega * (fneq[i] + fneq[ib]);
-      const double asym = a.half_omega_minus * (fneq[i] - fneq[ib]);
-      fpost[i] = f[i] + sym + asym;
-      fpost[ib] = f[ib] + sym - asym;
-    }
-  } else {
-    // LBGKLESSpongeLayer.h:150-182 with lifetime 0.
-    const Sym3 s = second_moments(fneq);
-    double q12 = s.xx * s.xx;
-    q12 += s.yx * s.yx;
-    q12 += s.zx * s.zx;
-    q12 += s.yx * s.yx;
-    q12 += s.yy * s.yy;
-    q12 += s.zy * s.zy;
-    q12 += s.zx * s.zx;
-    q12 += s.zy * s.zy;
-    q12 += s.zz * s.zz;
-    q12 = __builtin_sqrt(q12);
-    tau = 1. / 2. * (a.tau + __builtin_sqrt(a.tau * a.tau + a.les_k * q12));
-    const double om = -1.0 / tau;
-#pragma unroll
-    for (int j = 0; j < Q; ++j) fpost[j] = f[j] + fneq[j] * om;
-  }
+  collide_site<KERNEL>(a, f, rho, mx, my, mz, tau, fpost);
 
 #pragma unroll
   for (int j = 0; j < Q; ++j) stp(a.aout[j], d, fpost[j]);
@@ -303,6 +311,89 @@ __global__ __launch_bounds__(kBlock) void gather_fp64(StepArgs a, double* __rest
   if (L.boundary) boundary_fixup(a, L.slot, L.b, wm, im, f);
 #pragma unroll
   for (int j = 0; j < Q; ++j) stp(plane(F, S, j), L.slot, f[j]);
+}
+
+// The stress caches of MacroscopicPropertyCache (BaseStreamer.h:120-220)
+// from the pre-collision state F (FP64, or FP32-shifted g with f = g + w):
+// 17 planes of S doubles -- the full stress tensor (xx, xy, xz, yy, yz, zz;
+// Lattice.h:1096-1111), von Mises stress (:983-1001), shear rate
+// (:1159-1180), and at wall sites the traction (:1017-1027) with its
+// tangential (:1045-1056) and normal (:1069-1079) projections (zero vectors
+// elsewhere).  The collision and Pi tensor (:1353-1383) are the step's.
+template <int KERNEL, typename T>
+__global__ __launch_bounds__(kBlock) void stress_fp64(StepArgs a, const T* __restrict__ F, double* __restrict__ out) {
+  Lane L;
+  if (!slab_lane(a.bt, slab_record(), L)) return;
+  const uint32_t d = L.slot;
+  double f[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    if constexpr (sizeof(T) == 8) f[j] = (double)ldp(plane(F, a.S, j), d);
+    else f[j] = (double)ldp(plane(F, a.S, j), d) + Wt(j);
+  }
+  double rho, mx, my, mz;
+  moments(f, rho, mx, my, mz);
+  double tau;
+  double fpost[Q];
+  collide_site<KERNEL>(a, f, rho, mx, my, mz, tau, fpost);
+  double g[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) g[j] = fpost[j] - f[j];
+  const Sym3 s = second_moments(g);
+  const double fac = -(tau - 0.5);
+  const double p00 = s.xx * fac, p10 = s.yx * fac, p11 = s.yy * fac;
+  const double p20 = s.zx * fac, p21 = s.zy * fac, p22 = s.zz * fac;
+  const double pi[3][3] = {{p00, p10, p20}, {p10, p11, p21}, {p20, p21, p22}};
+  // stress tensor: -Pi, minus the pressure (rho - 1) Cs2 on the diagonal
+  double sig[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sig[r][c] = -1.0 * pi[r][c];
+  const double pressure = (rho - 1) * CS2;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) sig[r][r] += -pressure;
+  const double ten[6] = {sig[0][0], sig[0][1], sig[0][2], sig[1][1], sig[1][2], sig[2][2]};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) stp(plane(out, a.S, k), d, ten[k]);
+  // von Mises stress
+  const double dxy = p00 - p11, dyz = p11 - p22, dzx = p22 - p00;
+  const double t1 = dxy * dxy + dyz * dyz + dzx * dzx;
+  const double t2 = pi[0][1] * pi[0][1] + pi[1][2] * pi[1][2] + pi[2][0] * pi[2][0];
+  stp(plane(out, a.S, 6), d, __builtin_sqrt(0.5 * (t1 + 6.0 * t2)));
+  // shear rate
+  double sr = 0.0;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sr += pi[r][c] * pi[r][c];
+  sr = __builtin_sqrt(sr) / (-2.0 * CS2 * rho);
+  sr /= (tau - 0.5);
+  stp(plane(out, a.S, 7), d, sr);
+  // traction and its projections at wall sites
+  double tr[3] = {0.0, 0.0, 0.0}, tt[3] = {0.0, 0.0, 0.0}, tn[3] = {0.0, 0.0, 0.0};
+  if (L.boundary && ldp(a.bwall, L.b) != 0u) {
+    const double n[3] = {(double)ldp(a.bnrm, L.b), (double)ldp(a.bnrm + a.SB, L.b),
+                         (double)ldp(a.bnrm + 2 * a.SB, L.b)};
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      tr[r] = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) tr[r] += sig[r][c] * n[c];
+    }
+    const double mag = tr[0] * n[0] + tr[1] * n[1] + tr[2] * n[2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      tt[r] = tr[r] - n[r] * mag;
+      tn[r] = n[r] * mag;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    stp(plane(out, a.S, 8 + r), d, tr[r]);
+    stp(plane(out, a.S, 11 + r), d, tt[r]);
+    stp(plane(out, a.S, 14 + r), d, tn[r]);
+  }
 }
 
 // GuoZhengShiDelegate::StreamLink (GuoZhengShiDelegate.h:123-294), one
@@ -637,6 +728,21 @@ hipError_t launch_monitor(const BrickTables& bt, const double* rho, const double
   Mon* part = static_cast<Mon*>(scratch);
   hipLaunchKernelGGL(monitor_pass1, dim3(grid), dim3(kBlock), 0, s, bt, rho, u, S, part);
   hipLaunchKernelGGL(monitor_pass2, dim3(1), dim3(kBlock), 0, s, part, (int)grid, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_stress_fp64(const StepArgs& a, int kernel, const void* F, bool fp32, double* out, hipStream_t s) {
+  if (a.bt.nslab <= 0) return hipSuccess;
+  const dim3 grid(slab_grid(a.bt.nslab)), block(kBlock);
+#define HLB_S(KN)                                                                                          \
+  if (fp32) hipLaunchKernelGGL((stress_fp64<KN, float>), grid, block, 0, s, a, static_cast<const float*>(F), out); \
+  else hipLaunchKernelGGL((stress_fp64<KN, double>), grid, block, 0, s, a, static_cast<const double*>(F), out)
+  switch (kernel) {
+    case HLB_KERNEL_LBGK: HLB_S(HLB_KERNEL_LBGK); break;
+    case HLB_KERNEL_TRT: HLB_S(HLB_KERNEL_TRT); break;
+    default: HLB_S(HLB_KERNEL_LES); break;
+  }
+#undef HLB_S
   return hipGetLastError();
 }
 

@@ -193,6 +193,17 @@ int hlb_download_f(hlb_handle h, double* f);
  * HLB_STEP_PROPERTIES (src/lb/MacroscopicPropertyCache.h:24-106).  u is n*3;
  * WSS is DBL_MAX (NO_VALUE) at sites without a wall link.  Any may be NULL. */
 int hlb_get_fields(hlb_handle h, double* rho, double* u, double* wss);
+/* The stress caches of MacroscopicPropertyCache (src/lb/MacroscopicPropertyCache.h)
+ * that step T's collision fills (BaseStreamer::UpdateMinsAndMaxes,
+ * src/lb/streamers/BaseStreamer.h:120-220), computed on the device from the
+ * state entering step T = hlb_timestep(h), lattice units, site order:
+ * tensor n*6 (full stress tensor xx, xy, xz, yy, yz, zz: -Pi - (rho-1)Cs2 I,
+ * Lattice.h:1096-1111), von Mises stress n (:983-1001), shear rate n
+ * (:1159-1180), and n*3 each the traction, its tangential and its normal
+ * projection at wall sites (:1017-1079; zero vectors elsewhere).  Any may be
+ * NULL. */
+int hlb_get_stress(hlb_handle h, double* tensor, double* von_mises, double* shear_rate, double* traction,
+                   double* tangential, double* normal);
 /* SimulationState::GetTimeStep (src/lb/SimulationState.cc:50): the 1-indexed
  * step that runs next. */
 int64_t hlb_timestep(hlb_handle h);
@@ -201,15 +212,25 @@ int64_t hlb_timestep(hlb_handle h);
 int hlb_stability(hlb_handle h, int* stable);
 
 /* ---- property extraction files (.xtr + .off) ---------------------------- */
-/* Field types of OutputField (src/extraction/OutputField.h).  A field is
+/* Field types, numbered as OutputField::FieldType (src/extraction/
+ * OutputField.h:17-31).  A field is
  * written under its given name, or its type name ("pressure", "velocity",
  * "shearstress", "distributions", "mpirank") when none is given, as
- * SimConfig::DoIOForPropertyField (SimConfig.cc:565-580) defaults it. */
-#define HLB_FIELD_PRESSURE 0      /* mmHg, offset REFERENCE_PRESSURE_mmHg */
-#define HLB_FIELD_VELOCITY 1      /* m/s, 3 floats */
-#define HLB_FIELD_SHEAR_STRESS 2  /* Pa, wall sites; elsewhere inf */
-#define HLB_FIELD_DISTRIBUTIONS 3 /* 19 floats, lattice units */
-#define HLB_FIELD_MPI_RANK 4      /* 0: one process */
+ * SimConfig::DoIOForPropertyField (SimConfig.cc:565-580) defaults it.
+ * Wall extension has no source here (only the elastic GZS wall sets it) and
+ * is written as 0. */
+#define HLB_FIELD_PRESSURE 0                       /* mmHg, offset REFERENCE_PRESSURE_mmHg */
+#define HLB_FIELD_VELOCITY 1                       /* m/s, 3 floats */
+#define HLB_FIELD_SHEAR_STRESS 2                   /* Pa, wall sites; elsewhere inf */
+#define HLB_FIELD_VON_MISES_STRESS 3               /* Pa */
+#define HLB_FIELD_SHEAR_RATE 4                     /* 1/s */
+#define HLB_FIELD_STRESS_TENSOR 5                  /* Pa, xx xy xz yy yz zz */
+#define HLB_FIELD_TRACTION 6                       /* Pa, 3 floats; NaN where no normal */
+#define HLB_FIELD_TANGENTIAL_PROJECTION_TRACTION 7 /* Pa, 3 floats */
+#define HLB_FIELD_NORMAL_PROJECTION_TRACTION 8     /* Pa, 3 floats */
+#define HLB_FIELD_WALL_EXTENSION 9                 /* m; 0 (no elastic walls) */
+#define HLB_FIELD_DISTRIBUTIONS 10                 /* 19 floats, lattice units */
+#define HLB_FIELD_MPI_RANK 11                      /* 0: one process */
 /* Geometry selectors (the src/extraction/...Selector.cc classes). */
 #define HLB_SELECT_WHOLE 0
 #define HLB_SELECT_SURFACE 1 /* wall sites */
@@ -217,6 +238,9 @@ int hlb_stability(hlb_handle h, int* stable);
 #define HLB_SELECT_OUTLET 3  /* SiteType OUTLET_TYPE */
 #define HLB_SELECT_PLANE 4   /* |distance| <= voxel/2, within radius if > 0 */
 #define HLB_SELECT_SPHERE 5  /* within radius of point */
+#define HLB_SELECT_LINE 6    /* within voxel/sqrt2 of the segment point..point2 */
+#define HLB_SELECT_SURFACE_POINT 7  /* wall sites within sqrt3 voxels of point */
+#define HLB_SELECT_SURFACE_SPHERE 8 /* wall sites within radius of point */
 /* One <propertyoutput> of the XML (SimConfig::DoIOForPropertyOutputFile,
  * src/configuration/SimConfig.cc:420-560): the fields, the
  * selector with its point (m), normal and radius (m), and the unit
@@ -232,6 +256,7 @@ typedef struct hlb_xtr_spec {
   double point[3];
   double normal[3];
   double radius;
+  double point2[3]; /* the line's second end */
 } hlb_xtr_spec;
 typedef struct hlb_xtr* hlb_xtr_handle;
 /* LocalPropertyOutput::LocalPropertyOutput (src/extraction/
@@ -242,17 +267,19 @@ int hlb_xtr_open(hlb_handle h, const char* path, const hlb_xtr_spec* spec, hlb_x
 /* Record T holds what the reference writes at the end of step T
  * (PropertyActor::EndIteration, src/extraction/PropertyActor.cc:94-99): the
  * distributions that entered step T and step T's property caches.  Call
- * hlb_xtr_capture before step T (it copies f when a field needs it), run
+ * hlb_xtr_capture before step T (it copies f and computes the stress caches
+ * when a field needs them), run
  * step T with HLB_STEP_PROPERTIES, then hlb_xtr_write (LocalPropertyOutput::
  * Write, :216-327), which returns T.  A distributions-only record needs no
  * step in between. */
 int hlb_xtr_capture(hlb_xtr_handle x);
 int hlb_xtr_write(hlb_xtr_handle x, int64_t* timestep);
-/* Writes one record from caller arrays in the lattice's site order (rho n,
- * u n*3, wss n, f n*19, lattice units); arrays a field does not use may be
- * NULL. */
+/* Writes one record from caller arrays in the lattice's site order, lattice
+ * units: rho n, u n*3, wss n, f n*19, stress n*17 (per site the stress
+ * tensor's 6, von Mises, shear rate, traction 3, tangential 3, normal 3, as
+ * hlb_get_stress returns them); arrays no field uses may be NULL. */
 int hlb_xtr_write_fields(hlb_xtr_handle x, int64_t timestep, const double* rho, const double* u,
-                         const double* wss, const double* f);
+                         const double* wss, const double* f, const double* stress);
 int64_t hlb_xtr_site_count(hlb_xtr_handle x);
 void hlb_xtr_close(hlb_xtr_handle x);
 /* LocalDistributionInput::LoadDistribution (src/extraction/

@@ -417,6 +417,13 @@ struct OutputField {
     Pressure = HLB_FIELD_PRESSURE,
     Velocity = HLB_FIELD_VELOCITY,
     ShearStress = HLB_FIELD_SHEAR_STRESS,
+    VonMisesStress = HLB_FIELD_VON_MISES_STRESS,
+    ShearRate = HLB_FIELD_SHEAR_RATE,
+    StressTensor = HLB_FIELD_STRESS_TENSOR,
+    Traction = HLB_FIELD_TRACTION,
+    TangentialProjectionTraction = HLB_FIELD_TANGENTIAL_PROJECTION_TRACTION,
+    NormalProjectionTraction = HLB_FIELD_NORMAL_PROJECTION_TRACTION,
+    WallExtension = HLB_FIELD_WALL_EXTENSION,
     Distributions = HLB_FIELD_DISTRIBUTIONS,
     MpiRank = HLB_FIELD_MPI_RANK
   };
@@ -433,6 +440,7 @@ struct PropertyOutputFile {
   std::vector<OutputField> fields;
   int32_t selector = HLB_SELECT_WHOLE;
   double point[3] = {0, 0, 0}, normal[3] = {0, 0, 1}, radius = 0;
+  double point2[3] = {0, 0, 0};  // second end of a line selector
 };
 
 /* LocalPropertyOutput (src/extraction/LocalPropertyOutput.cc): opening
@@ -460,12 +468,15 @@ class LocalPropertyOutput {
       x.origin[a] = origin[a];
       x.point[a] = spec.point[a];
       x.normal[a] = spec.normal[a];
+      x.point2[a] = spec.point2[a];
     }
     x.radius = spec.radius;
     hlb_xtr_handle h = nullptr;
     detail::Check(hlb_xtr_open(latDat.Handle(), spec.filename.c_str(), &x, &h));
     handle.reset(h);
-    for (const auto& f : spec.fields) needsProperties = needsProperties || f.type != OutputField::Distributions;
+    for (const auto& f : spec.fields)
+      needsProperties = needsProperties || f.type == OutputField::Pressure || f.type == OutputField::Velocity ||
+                        f.type == OutputField::ShearStress;
   }
   bool ShouldWrite(uint64_t t) const {
     return t >= outputSpec.start && t <= outputSpec.stop && t % outputSpec.frequency == 0;

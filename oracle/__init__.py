@@ -96,6 +96,8 @@ def lib() -> C.CDLL:
             "orc_collide": (None, [C.c_int, C.c_double, C.c_double, vp, C.c_int, vp, vp]),
             "orc_wss": (C.c_double, [C.c_double, C.c_double, vp, vp, vp]),
             "orc_cosine_density": (C.c_double, [C.POINTER(OrcIolet), C.c_double, C.c_uint64]),
+            "orc_stress_site": (None, [C.c_int, C.c_double, C.c_double, vp, C.c_int, vp, C.c_int, vp]),
+            "orc_get_stress": (None, [vp, vp]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -231,6 +233,14 @@ class OracleLBM:
         lib().orc_get_props(self._h, _p(rho), _p(u), _p(wss))
         return rho, u, wss
 
+    def stress(self):
+        """Stress caches from the state entering the next step, n x 17: tensor
+        (xx, xy, xz, yy, yz, zz), von Mises, shear rate, traction, tangential
+        and normal projections."""
+        out = np.zeros((self.n, 17))
+        lib().orc_get_stress(self._h, _p(out))
+        return out
+
     def timestep(self):
         return int(lib().orc_timestep(self._h))
 
@@ -270,6 +280,14 @@ def wss(rho, tau, fpost, f, n):
     f = np.ascontiguousarray(f, dtype=np.float64)
     n = np.ascontiguousarray(n, dtype=np.float64)
     return float(lib().orc_wss(rho, tau, _p(fpost), _p(f), _p(n)))
+
+
+def stress_site(kernel, tau, lam, f, n, wall=True, sse3=True):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    out = np.zeros(17)
+    lib().orc_stress_site(kernel, tau, lam, _p(f), 1 if sse3 else 0, _p(n), 1 if wall else 0, _p(out))
+    return out
 
 
 def cosine_density(io: OrcIolet, min_density: float, t0: int) -> float:

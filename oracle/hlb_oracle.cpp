@@ -202,11 +202,10 @@ inline void Collide(int kernel, double tau0, double omega, double lambda, const 
   }
 }
 
-// Wall shear stress magnitude, src/lb/lattices/Lattice.h:1122-1156 with the
-// Pi tensor of :1353-1383.
-inline double WallShearStress(double tau, const double* fpost, const double* f, const double nor[3]) {
+// Non-equilibrium momentum flux, src/lb/lattices/Lattice.h:1353-1383 (lower
+// triangle summed, scaled, mirrored).
+inline void PiTensor(double tau, const double* fpost, const double* f, double pi[3][3]) {
   const int* cv[3] = {CX, CY, CZ};
-  double pi[3][3];
   for (int ii = 0; ii < 3; ++ii) {
     for (int jj = 0; jj <= ii; ++jj) {
       pi[ii][jj] = 0.0;
@@ -216,6 +215,12 @@ inline double WallShearStress(double tau, const double* fpost, const double* f, 
   }
   for (int ii = 0; ii < 3; ++ii)
     for (int jj = ii + 1; jj < 3; ++jj) pi[ii][jj] = pi[jj][ii];
+}
+
+// Wall shear stress magnitude, src/lb/lattices/Lattice.h:1122-1156.
+inline double WallShearStress(double tau, const double* fpost, const double* f, const double nor[3]) {
+  double pi[3][3];
+  PiTensor(tau, fpost, f, pi);
   double sv[3] = {0.0, 0.0, 0.0};
   double square = 0.0, normal = 0.0;
   for (int i = 0; i < 3; i++) {
@@ -224,6 +229,57 @@ inline double WallShearStress(double tau, const double* fpost, const double* f, 
     normal += sv[i] * nor[i];
   }
   return std::sqrt(square - normal * normal);
+}
+
+// The stress caches of one site (BaseStreamer.h:120-220), 17 values: the
+// full stress tensor's upper triangle row-wise (CalculateStressTensor,
+// Lattice.h:1096-1111, Matrix3D * -1.0 then addDiagonal), von Mises stress
+// (:983-1001), shear rate (:1159-1180), then, when `wall`, the traction
+// (:1017-1027, Matrix3D::timesVector), its tangential (:1045-1056) and normal
+// (:1069-1079) projections; zero vectors otherwise.
+inline void StressFields(double rho, double tau, const double* fpost, const double* f, const double nor[3],
+                         bool wall, double* out) {
+  double pi[3][3];
+  PiTensor(tau, fpost, f, pi);
+  double sig[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) sig[r][c] = -1.0 * pi[r][c];
+  const double pressure = (rho - 1) * Cs2;
+  for (int r = 0; r < 3; ++r) sig[r][r] += -pressure;
+  out[0] = sig[0][0];
+  out[1] = sig[0][1];
+  out[2] = sig[0][2];
+  out[3] = sig[1][1];
+  out[4] = sig[1][2];
+  out[5] = sig[2][2];
+  const double sxx_yy = pi[0][0] - pi[1][1], syy_zz = pi[1][1] - pi[2][2], szz_xx = pi[2][2] - pi[0][0];
+  const double sxy = pi[0][1], syz = pi[1][2], szx = pi[2][0];
+  const double temp1 = sxx_yy * sxx_yy + syy_zz * syy_zz + szz_xx * szz_xx;
+  const double temp2 = sxy * sxy + syz * syz + szx * szx;
+  out[6] = std::sqrt(0.5 * (temp1 + 6.0 * temp2));
+  double rate = 0.0;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) rate += pi[r][c] * pi[r][c];
+  rate = std::sqrt(rate) / (-2.0 * Cs2 * rho);
+  rate /= (tau - 0.5);
+  out[7] = rate;
+  double t[3] = {0.0, 0.0, 0.0}, tan[3] = {0.0, 0.0, 0.0}, nrm[3] = {0.0, 0.0, 0.0};
+  if (wall) {
+    for (int r = 0; r < 3; ++r) {
+      t[r] = 0.0;
+      for (int c = 0; c < 3; ++c) t[r] += sig[r][c] * nor[c];
+    }
+    const double mag = t[0] * nor[0] + t[1] * nor[1] + t[2] * nor[2];
+    for (int r = 0; r < 3; ++r) {
+      tan[r] = t[r] - nor[r] * mag;
+      nrm[r] = nor[r] * mag;
+    }
+  }
+  for (int r = 0; r < 3; ++r) {
+    out[8 + r] = t[r];
+    out[11 + r] = tan[r];
+    out[14 + r] = nrm[r];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -942,6 +998,23 @@ extern "C" void orc_collide(int kernel, double tau, double lambda, const double*
 extern "C" double orc_wss(double rho, double tau, const double* fpost, const double* f, const double n[3]) {
   (void)rho;
   return WallShearStress(tau, fpost, f, n);
+}
+extern "C" void orc_stress_site(int kernel, double tau, double lambda, const double* f, int sse3, const double n[3],
+                                int wall, double* out) {
+  Hydro h;
+  PreCollision(sse3 != 0, kernel, tau, f, h);
+  Collide(kernel, tau, -1.0 / tau, lambda, f, h);
+  StressFields(h.rho, h.tau, h.fpost, f, n, wall != 0, out);
+}
+extern "C" void orc_get_stress(const orc_lbm* m, double* out) {
+  const orc_lattice* L = m->L;
+  for (int64_t k = 0; k < L->N; ++k) {
+    const double* f = &m->fOld[k * Q];
+    Hydro h;
+    PreCollision(m->p.sse3_order != 0, m->p.kernel, m->p.tau, f, h);
+    Collide(m->p.kernel, m->p.tau, m->omega, m->p.trt_lambda, f, h);
+    StressFields(h.rho, h.tau, h.fpost, f, &L->normal[3 * k], L->site[k].wall != 0, out + 17 * k);
+  }
 }
 extern "C" double orc_cosine_density(const orc_iolet* io, double min_density, uint64_t t0) {
   return CosineDensity(*io, min_density, t0);

@@ -129,6 +129,13 @@ void orc_collide(int kernel, double tau, double lambda, const double* f, int sse
                  double* tau_out);
 double orc_wss(double rho, double tau, const double* fpost, const double* f, const double n[3]);
 double orc_cosine_density(const orc_iolet* io, double min_density, uint64_t t0);
+/* Stress caches of one site from its pre-collision f (17 values: stress
+ * tensor xx xy xz yy yz zz, von Mises, shear rate, traction, tangential and
+ * normal projections; the last nine zero unless wall). */
+void orc_stress_site(int kernel, double tau, double lambda, const double* f, int sse3, const double n[3],
+                     int wall, double* out);
+/* The same for every site from the state entering the next step, n*17. */
+void orc_get_stress(const orc_lbm* m, double* out);
 
 #ifdef __cplusplus
 }

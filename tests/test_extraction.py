@@ -321,3 +321,73 @@ def test_selectors(tmp_path):
     ob = selected(lat, tmp_path, "oblique", selector=H.SELECT_PLANE, point=(3.5, 3.5, 0.0), normal=(1.0, 1.0, 0.0))
     dist = np.abs(coords[:, 0] + coords[:, 1] - 7.0) / np.sqrt(2.0)
     assert np.array_equal(ob, coords[dist <= 0.5])
+
+
+def test_line_and_surface_selectors(tmp_path):
+    g = four_cube(8)  # fluid 1..6; x/y faces are walls
+    lat = H.Lattice(g, None)
+    coords = lat.site_coords()
+    wall, _, _, _ = lat.site_masks()
+    # a line along x through (y, z) = (3.5, 3.5): within sqrt(0.5) voxel of it
+    ln = selected(lat, tmp_path, "line", selector=H.SELECT_LINE, point=(1.0, 3.5, 3.5), point2=(6.0, 3.5, 3.5))
+    d2 = (coords[:, 1] - 3.5) ** 2 + (coords[:, 2] - 3.5) ** 2
+    assert np.array_equal(ln, coords[d2 <= 0.5])
+    # a shorter, oblique segment: the projection must fall on the segment
+    ob = selected(lat, tmp_path, "oblique_line", selector=H.SELECT_LINE, point=(2.0, 2.0, 3.0),
+                  point2=(5.0, 5.0, 3.0))
+    p0, v = np.array([2.0, 2.0, 3.0]), np.array([3.0, 3.0, 0.0])
+    t = (coords - p0) @ v / np.linalg.norm(v)
+    foot = p0 + np.outer(t / np.linalg.norm(v), v)
+    dist2 = ((foot - coords) ** 2).sum(axis=1)
+    assert np.array_equal(ob, coords[(t >= 0) & (t <= np.linalg.norm(v)) & (dist2 <= 0.5)])
+    # wall sites within sqrt(3) voxels of a point on the x = 1 face
+    sp = selected(lat, tmp_path, "surfpoint", selector=H.SELECT_SURFACE_POINT, point=(1.0, 3.0, 3.0))
+    near = ((coords - np.array([1, 3, 3])) ** 2).sum(axis=1) <= 3
+    assert np.array_equal(sp, coords[near & (wall != 0)])
+    assert len(sp) > 0
+    # wall sites within a sphere
+    ss = selected(lat, tmp_path, "surfsphere", selector=H.SELECT_SURFACE_SPHERE, point=(3.0, 1.0, 4.0), radius=2.5)
+    inside = ((coords - np.array([3, 1, 4])) ** 2).sum(axis=1) <= 2.5 ** 2
+    assert np.array_equal(ss, coords[inside & (wall != 0)])
+    assert 0 < len(ss) < inside.sum()
+    with pytest.raises(H.HlbError, match="zero length"):
+        H.Extraction(lat, tmp_path / "bad.xtr", [H.FIELD_PRESSURE], selector=H.SELECT_LINE, point=(1, 1, 1),
+                     point2=(1, 1, 1))
+
+
+def test_stress_fields_units_and_layout(tmp_path):
+    """The stress-derived fields in physical units: stresses scale by the
+    lattice pressure rho_blood (dx/dt)^2, the shear rate by 1/dt; the tensor
+    is written as its upper triangle row-wise (6 floats); traction where the
+    site has no wall normal is NaN, as the reference's conversion makes it
+    (inf normal times the zero reference pressure); wall extension is 0."""
+    g = four_cube(8)
+    lat = H.Lattice(g, None)
+    n = g.n_sites
+    rng = np.random.default_rng(11)
+    st = rng.uniform(-1e-3, 1e-3, size=(n, 17))
+    dx, dt = 2e-4, 5e-6
+    lp = 1000.0 * (dx / dt) ** 2
+    fields = [H.FIELD_STRESS_TENSOR, H.FIELD_VON_MISES_STRESS, H.FIELD_SHEAR_RATE, H.FIELD_TRACTION,
+              H.FIELD_TANGENTIAL_PROJECTION_TRACTION, H.FIELD_NORMAL_PROJECTION_TRACTION, H.FIELD_WALL_EXTENSION]
+    path = tmp_path / "stress.xtr"
+    with H.Extraction(lat, path, fields, voxel_size=dx, time_step=dt) as x:
+        x.write_fields(5, stress=st)
+    f = Xtr(path)
+    assert [(a, b) for a, b, _ in f.fields] == [
+        ("stresstensor", 6), ("vonmisesstress", 1), ("shearrate", 1), ("traction", 3),
+        ("tangentialprojectiontraction", 3), ("normalprojectiontraction", 3), ("wallextension", 1)]
+    t, coords, vals = f.records[0]
+    assert t == 5
+    rel = dict(rtol=2e-7, atol=0)
+    np.testing.assert_allclose(vals["stresstensor"], st[:, :6] * lp, **rel)
+    np.testing.assert_allclose(vals["vonmisesstress"][:, 0], st[:, 6] * lp, **rel)
+    np.testing.assert_allclose(vals["shearrate"][:, 0], st[:, 7] / dt, **rel)
+    _, normals = lat.wall_geometry()
+    has_normal = np.isfinite(normals).all(axis=1)
+    assert has_normal.any() and (~has_normal).any()
+    np.testing.assert_allclose(vals["traction"][has_normal], st[has_normal, 8:11] * lp, **rel)
+    assert np.isnan(vals["traction"][~has_normal]).all()
+    np.testing.assert_allclose(vals["tangentialprojectiontraction"], st[:, 11:14] * lp, **rel)
+    np.testing.assert_allclose(vals["normalprojectiontraction"], st[:, 14:17] * lp, **rel)
+    assert (vals["wallextension"] == 0).all()

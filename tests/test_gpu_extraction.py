@@ -15,14 +15,16 @@ from fixtures import cylinder_case, four_cube, Case, IoletSpec
 
 pytestmark = pytest.mark.gpu
 
-FIELDS = [H.FIELD_PRESSURE, H.FIELD_VELOCITY, H.FIELD_SHEAR_STRESS, H.FIELD_DISTRIBUTIONS]
+FIELDS = [H.FIELD_PRESSURE, H.FIELD_VELOCITY, H.FIELD_SHEAR_STRESS, H.FIELD_VON_MISES_STRESS, H.FIELD_SHEAR_RATE,
+          H.FIELD_STRESS_TENSOR, H.FIELD_TRACTION, H.FIELD_TANGENTIAL_PROJECTION_TRACTION,
+          H.FIELD_NORMAL_PROJECTION_TRACTION, H.FIELD_WALL_EXTENSION, H.FIELD_DISTRIBUTIONS, H.FIELD_MPI_RANK]
 
 
 def pulsatile_cylinder():
     return cylinder_case(radius=6.0, length=40, box_xy=16, rho_in=1.002, rho_out=0.999, amp=5e-4, period=50.0)
 
 
-@pytest.mark.parametrize("selector", [H.SELECT_WHOLE, H.SELECT_SURFACE, H.SELECT_PLANE])
+@pytest.mark.parametrize("selector", [H.SELECT_WHOLE, H.SELECT_SURFACE, H.SELECT_PLANE, H.SELECT_LINE])
 def test_record_matches_oracle_record(tmp_path, selector):
     case = pulsatile_cylinder()
     T = 37
@@ -32,6 +34,8 @@ def test_record_matches_oracle_record(tmp_path, selector):
     ref.init_equilibrium(1.0, 1)
     kw = dict(voxel_size=5e-5, time_step=1e-5, origin=(0.001, 0.002, -0.003), selector=selector,
               point=(0.001 + 8 * 5e-5, 0.0, 0.0), normal=(1.0, 0.0, 0.0))
+    if selector == H.SELECT_LINE:  # along the axis
+        kw.update(point=(0.001 + 8 * 5e-5, 0.002 + 8 * 5e-5, -0.003), point2=(0.001 + 8 * 5e-5, 0.002 + 8 * 5e-5, -0.001))
     dev = H.Extraction(lat, tmp_path / "dev.xtr", FIELDS, **kw)
     want = H.Extraction(lat, tmp_path / "want.xtr", FIELDS, **kw)
     assert dev.site_count > 0
@@ -41,11 +45,12 @@ def test_record_matches_oracle_record(tmp_path, selector):
         assert lat.timestep() == ref.timestep() == t_rec
         dev.capture()
         f_in = ref.f()
+        st_in = ref.stress()
         lat.step(1, properties=True)
         ref.run(1, properties=True, threads=0)
         assert dev.write() == t_rec
         rho, u, wss = ref.fields()
-        want.write_fields(t_rec, rho=rho, u=u, wss=wss, f=f_in)
+        want.write_fields(t_rec, rho=rho, u=u, wss=wss, f=f_in, stress=st_in)
     dev.close()
     want.close()
     a = open(tmp_path / "dev.xtr", "rb").read()
@@ -125,3 +130,35 @@ def test_inlet_outlet_selectors_on_device_lattice(tmp_path):
         dev.close()
         want.close()
         assert open(tmp_path / f"d{sel}.xtr", "rb").read() == open(tmp_path / f"w{sel}.xtr", "rb").read()
+
+
+@pytest.mark.parametrize("kernel", [H.KERNEL_LBGK, H.KERNEL_TRT, H.KERNEL_LES])
+def test_stress_caches_match_oracle(kernel):
+    """hlb_get_stress (device) against the oracle's restatement of the
+    reference's stress caches, bit for bit, from the same state."""
+    case = pulsatile_cylinder()
+    case.kernel = kernel
+    lat = case.product()
+    ref = case.oracle()
+    lat.init_equilibrium(1.0, 1)
+    ref.init_equilibrium(1.0, 1)
+    lat.step(60)
+    ref.run(60, threads=0)
+    got, want = lat.stress(), ref.stress()
+    both_nan = np.isnan(got) & np.isnan(want)
+    assert np.array_equal(np.where(both_nan, 0, got), np.where(both_nan, 0, want))
+    assert np.abs(want[:, 6]).max() > 0 and np.abs(want[:, 8:11]).max() > 0
+
+
+def test_stress_caches_fp32_close_to_fp64():
+    case = pulsatile_cylinder()
+    a = case.product(H.FP32_SHIFTED)
+    b = case.product()
+    a.init_equilibrium(1.0, 1)
+    b.init_equilibrium(1.0, 1)
+    a.step(60)
+    b.step(60)
+    sa, sb = a.stress(), b.stress()
+    fin = np.isfinite(sb)
+    scale = np.abs(sb[fin]).max()
+    assert np.abs(sa[fin] - sb[fin]).max() <= 1e-4 * scale

@@ -134,6 +134,77 @@ def test_wall_shear_stress_closed_form():
     np.testing.assert_allclose(got_x, (tau - 0.5) * 4 * a, rtol=1e-12)
 
 
+def shear_state(rho, a):
+    """f_eq(rho, 0) plus a pure x-y shear non-equilibrium part (sum c_x c_y g =
+    4a, every other moment up to second order zero)."""
+    g = np.zeros(19)
+    g[7] = g[8] = a
+    g[9] = g[10] = -a
+    return O.feq(rho, 0.0, 0.0, 0.0) + g
+
+
+@pytest.mark.parametrize("kernel", [H.KERNEL_LBGK, H.KERNEL_TRT, H.KERNEL_LES])
+def test_stress_fields_pure_shear_closed_form(kernel):
+    """Lattice.h:983-1111, 1159-1180 on a pure shear: with f_post - f = -g/tau
+    (LBGK; TRT's symmetric pair relaxes the same way; LES with its own tau),
+    Pi_xy = 4a (tau - 1/2) / tau and every other Pi entry vanishes, so the
+    stress tensor is -Pi - (rho - 1)/3 I, von Mises sqrt(3) |Pi_xy|, the shear
+    rate -sqrt(2) |Pi_xy| / (2 Cs2 rho (tau - 1/2)); on a wall with normal y
+    the traction is (-Pi_xy, -p, 0), split into (-Pi_xy, 0, 0) and (0, -p, 0)."""
+    tau0, rho, a = 0.8, 1.02, 3e-5
+    f = shear_state(rho, a)
+    _, tau = O.collide(kernel, tau0, 3.0 / 16.0, f)
+    if kernel != H.KERNEL_LES:
+        assert tau == tau0
+    P = 4 * a * (tau - 0.5) / tau
+    p = (rho - 1) / 3
+    n = np.array([0.0, 1.0, 0.0])
+    out = O.stress_site(kernel, tau0, 3.0 / 16.0, f, n, wall=True)
+    tol = dict(rtol=1e-9, atol=1e-15)
+    np.testing.assert_allclose(out[:6], [-p, -P, 0.0, -p, 0.0, -p], **tol)
+    np.testing.assert_allclose(out[6], math.sqrt(3.0) * P, **tol)
+    np.testing.assert_allclose(out[7], -math.sqrt(2.0) * P / (2.0 / 3.0 * rho * (tau - 0.5)), **tol)
+    np.testing.assert_allclose(out[8:11], [-P, -p, 0.0], **tol)
+    np.testing.assert_allclose(out[11:14], [-P, 0.0, 0.0], **tol)
+    np.testing.assert_allclose(out[14:17], [0.0, -p, 0.0], **tol)
+    away = O.stress_site(kernel, tau0, 3.0 / 16.0, f, n, wall=False)
+    assert np.array_equal(away[:8], out[:8]) and not away[8:].any()
+
+
+def test_stress_fields_at_equilibrium():
+    rho = 0.97
+    f = O.feq(rho, 0.01, -0.02, 0.005)
+    out = O.stress_site(H.KERNEL_LBGK, 0.7, 3.0 / 16.0, f, np.array([0.6, 0.0, 0.8]))
+    p = (rho - 1) / 3
+    np.testing.assert_allclose(out[:6], [-p, 0, 0, -p, 0, -p], atol=1e-15)
+    assert abs(out[6]) < 1e-15 and abs(out[7]) < 1e-12
+    np.testing.assert_allclose(out[8:11], -p * np.array([0.6, 0.0, 0.8]), atol=1e-15)
+
+
+def test_stress_projections_agree_with_wall_shear_stress():
+    """On random near-equilibrium states: the tangential traction's magnitude
+    is the pinned wall shear stress (the pressure adds only a normal part),
+    tangential + normal = traction, tangential is orthogonal to the normal."""
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        f = O.feq(1.0 + 0.01 * rng.standard_normal(), *(0.01 * rng.standard_normal(3))) + \
+            1e-4 * rng.standard_normal(19)
+        n = rng.standard_normal(3)
+        n /= np.linalg.norm(n)
+        tau = 0.6 + rng.random()
+        out = O.stress_site(H.KERNEL_LBGK, tau, 3.0 / 16.0, f, n)
+        fpost, _ = O.collide(H.KERNEL_LBGK, tau, 3.0 / 16.0, f)
+        w = O.wss(1.0, tau, fpost, f, n)
+        tr, tt, tn = out[8:11], out[11:14], out[14:17]
+        np.testing.assert_allclose(np.linalg.norm(tt), w, rtol=1e-8)
+        np.testing.assert_allclose(tt + tn, tr, rtol=1e-12, atol=1e-18)
+        assert abs(tt @ n) < 1e-12 * np.linalg.norm(tr)
+        # the traction is the tensor times the normal
+        t6 = out[:6]
+        sig = np.array([[t6[0], t6[1], t6[2]], [t6[1], t6[3], t6[4]], [t6[2], t6[4], t6[5]]])
+        np.testing.assert_allclose(sig @ n, tr, rtol=1e-12, atol=1e-18)
+
+
 def test_cosine_iolet_formula():
     """InOutLetCosine::GetDensity (InOutLetCosine.cc:37-54) and the warm-up."""
     io = O.iolet(H.COSINE, (0, 0, 0), (0, 0, 1), density_mean=1.0, density_amp=1e-3, phase=0.3, period=400.0)
