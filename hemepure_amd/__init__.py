@@ -20,7 +20,7 @@ LINK_NONE, LINK_WALL, LINK_INLET, LINK_OUTLET = 0, 1, 2, 3
 KERNEL_LBGK, KERNEL_TRT, KERNEL_LES = 0, 1, 2
 WALL_SBB, WALL_BFL, WALL_GZS = 0, 1, 2
 IOLET_NASH, IOLET_LADD = 0, 1
-COSINE, PARABOLIC, FILEVEL, WK2, WK3 = 0, 1, 2, 3, 4
+COSINE, PARABOLIC, FILEVEL, WK2, WK3, FILEPRES, WOMERSLEY = 0, 1, 2, 3, 4, 5, 6
 FP64, FP32_SHIFTED = 0, 1
 STEP_PROPERTIES = 1
 (FIELD_PRESSURE, FIELD_VELOCITY, FIELD_SHEAR_STRESS, FIELD_VON_MISES_STRESS, FIELD_SHEAR_RATE,
@@ -66,6 +66,9 @@ class IoletDesc(C.Structure):
         ("max_speed", C.c_double),
         ("table", C.POINTER(C.c_double)),
         ("table_len", C.c_int64),
+        ("density_min", C.c_double),
+        ("pressure_gradient_amplitude", C.c_double),
+        ("womersley_number", C.c_double),
         ("area", C.c_double),
         ("resistance", C.c_double),
         ("r_c", C.c_double),
@@ -194,6 +197,8 @@ _SIGS = {
     "hlb_xtr_close": (None, [C.c_void_p]),
     "hlb_checkpoint_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.POINTER(C.c_int64),
                                       C.c_void_p]),
+    "hlb_iolet_file_table": (C.c_int, [C.c_char_p, C.c_int, C.c_int64, C.c_double, C.c_double, C.c_void_p,
+                                       C.c_int64, C.POINTER(C.c_double)]),
     "hlb_site_count": (C.c_int64, [C.c_void_p]),
     "hlb_class_counts": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hlb_site_coords": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -335,7 +340,8 @@ class Geometry:
 
 def iolet_desc(kind: int, position, normal, radius=0.0, density_mean=1.0, density_amp=0.0, phase=0.0,
                period=1.0, warmup=0, max_speed=0.0, table=None, area=1.0, resistance=1.0, r_c=1.0,
-               r_p=1.0, capacitance=1.0) -> IoletDesc:
+               r_p=1.0, capacitance=1.0, density_min=1.0, pressure_gradient_amplitude=0.0,
+               womersley_number=0.0) -> IoletDesc:
     d = IoletDesc()
     d.kind = kind
     d.warmup = warmup
@@ -352,12 +358,25 @@ def iolet_desc(kind: int, position, normal, radius=0.0, density_mean=1.0, densit
         d._table_keepalive = t
         d.table = t.ctypes.data_as(C.POINTER(C.c_double))
         d.table_len = t.size
+    d.density_min = density_min
+    d.pressure_gradient_amplitude = pressure_gradient_amplitude
+    d.womersley_number = womersley_number
     d.area = area
     d.resistance = resistance
     d.r_c = r_c
     d.r_p = r_p
     d.capacitance = capacitance
     return d
+
+
+def iolet_file_table(path, velocity: bool, total_steps: int, time_step: float, voxel_size: float):
+    """InOutLetFile / InOutLetFileVelocity::CalculateTable: (table, density_min)
+    with table[0..total_steps] in lattice density or lattice speed."""
+    out = np.empty(int(total_steps) + 1, np.float64)
+    dmin = C.c_double()
+    _check(lib().hlb_iolet_file_table(str(path).encode(), 1 if velocity else 0, int(total_steps), time_step,
+                                      voxel_size, _ptr(out), out.size, C.byref(dmin)))
+    return out, float(dmin.value)
 
 
 def params(kernel=KERNEL_LBGK, wall_bc=WALL_SBB, inlet_bc=IOLET_NASH, outlet_bc=IOLET_NASH, tau=0.8,

@@ -2,22 +2,28 @@
 
 Covers the parts of SimConfig (src/configuration/SimConfig.cc) that feed the
 collide-and-stream path: the time step and voxel size (relaxation time,
-LbmParameters.h:28-37), the geometry file, the initial pressure, and cosine
-pressure / parabolic velocity iolets with their position and normal.
-Pressures convert with UnitConverter (src/util/UnitConverter.cc:15-45):
-lattice pressure = Cs2 + dp[mmHg] * 133.3223874 / (1000 dx^2 / dt^2), and the
-iolet densities are pressure / Cs2 (InOutLetCosine.h:64-80).  The XML is read
+LbmParameters.h:28-37), the geometry file, the initial condition (uniform
+pressure or a checkpoint file, SimConfig.cc:668-705), every inlet/outlet
+condition the path implements (pressure: cosine, file, WK2, WK3, fileWK;
+velocity: parabolic, Womersley, file; SimConfig.cc:268-405, 725-990) and the
+property outputs (SimConfig.cc:410-630).  Values convert with UnitConverter
+(src/util/UnitConverter.cc:15-120, UnitConverter.h:150-203).  The XML is read
 with xml.etree (data only).
 """
 from __future__ import annotations
 
-import math
 import re
 import xml.etree.ElementTree as ET
 from dataclasses import dataclass, field
 from pathlib import Path
 
-from . import COSINE, PARABOLIC, Geometry, iolet_desc, params
+from . import (COSINE, FILEPRES, FILEVEL, PARABOLIC, WK2, WK3, WOMERSLEY, Geometry, iolet_desc, iolet_file_table,
+               params)
+from . import (FIELD_DISTRIBUTIONS, FIELD_MPI_RANK, FIELD_NORMAL_PROJECTION_TRACTION, FIELD_PRESSURE,
+               FIELD_SHEAR_RATE, FIELD_SHEAR_STRESS, FIELD_STRESS_TENSOR, FIELD_TANGENTIAL_PROJECTION_TRACTION,
+               FIELD_TRACTION, FIELD_VELOCITY, FIELD_VON_MISES_STRESS, FIELD_WALL_EXTENSION)
+from . import (SELECT_INLET, SELECT_LINE, SELECT_OUTLET, SELECT_PLANE, SELECT_SPHERE, SELECT_SURFACE,
+               SELECT_SURFACE_POINT, SELECT_SURFACE_SPHERE, SELECT_WHOLE, Extraction)
 
 CS2 = 1.0 / 3.0
 MMHG_TO_PASCAL = 133.3223874         # src/constants.h:24
@@ -25,11 +31,25 @@ BLOOD_DENSITY = 1000.0               # src/constants.h:25
 BLOOD_VISCOSITY = 0.004              # src/constants.h:27
 REFERENCE_PRESSURE_MMHG = 0.0        # src/constants.h:23
 
+# SimConfig::DoIOForPropertyField (SimConfig.cc:565-630)
+FIELD_TYPES = {
+    "pressure": FIELD_PRESSURE, "velocity": FIELD_VELOCITY, "vonmisesstress": FIELD_VON_MISES_STRESS,
+    "shearstress": FIELD_SHEAR_STRESS, "shearrate": FIELD_SHEAR_RATE, "stresstensor": FIELD_STRESS_TENSOR,
+    "traction": FIELD_TRACTION, "tangentialprojectiontraction": FIELD_TANGENTIAL_PROJECTION_TRACTION,
+    "normalprojectiontraction": FIELD_NORMAL_PROJECTION_TRACTION, "wallextension": FIELD_WALL_EXTENSION,
+    "distributions": FIELD_DISTRIBUTIONS, "mpirank": FIELD_MPI_RANK,
+}
 
-def _value(el, name, units=None):
+
+def _child(el, name):
     child = el.find(name)
     if child is None:
         raise ValueError(f"<{el.tag}> has no <{name}>")
+    return child
+
+
+def _value(el, name, units=None):
+    child = _child(el, name)
     if units is not None and child.get("units") not in (units, None):
         raise ValueError(f"<{name}> in {child.get('units')}, expected {units}")
     return child.get("value")
@@ -44,14 +64,18 @@ def _vector(text: str):
 
 @dataclass
 class UnitConverter:
-    """UnitConverter (src/util/UnitConverter.cc:15-45)."""
+    """UnitConverter (src/util/UnitConverter.cc:15-120)."""
     time_step: float
     voxel_size: float
+    origin: tuple = (0.0, 0.0, 0.0)
+
+    @property
+    def lattice_mass(self) -> float:
+        return BLOOD_DENSITY * self.voxel_size ** 3
 
     @property
     def lattice_pressure(self) -> float:
-        mass = BLOOD_DENSITY * self.voxel_size ** 3
-        return mass / (self.voxel_size * self.time_step ** 2)
+        return self.lattice_mass / (self.voxel_size * self.time_step ** 2)
 
     def pressure_difference(self, mmhg: float) -> float:
         return mmhg * MMHG_TO_PASCAL / self.lattice_pressure
@@ -65,6 +89,43 @@ class UnitConverter:
     def speed(self, m_per_s: float) -> float:
         return m_per_s / (self.voxel_size / self.time_step)
 
+    def distance(self, metres: float) -> float:
+        return metres / self.voxel_size
+
+    def area(self, m2: float) -> float:
+        return m2 / (self.voxel_size * self.voxel_size)
+
+    def resistance(self, r: float) -> float:
+        return r * ((self.time_step * self.voxel_size) / BLOOD_DENSITY)
+
+    def capacitance(self, c: float) -> float:
+        return c * (BLOOD_DENSITY / (self.time_step * self.time_step * self.voxel_size))
+
+    def pressure_gradient(self, mmhg_per_m: float) -> float:
+        # ConvertToLatticeUnits("mmHg/m"), UnitConverter.h:190-193
+        scale = self.lattice_mass / (self.voxel_size * self.voxel_size * self.time_step * self.time_step) \
+            / MMHG_TO_PASCAL
+        return mmhg_per_m / scale
+
+
+@dataclass
+class PropertyOutput:
+    """One <propertyoutput> (PropertyOutputFile, SimConfig.cc:420-490)."""
+    file: str
+    period: int
+    start: int = 0
+    stop: int = 2 ** 64 - 1
+    fields: list = field(default_factory=list)   # (type, name)
+    selector: int = SELECT_WHOLE
+    point: tuple = (0.0, 0.0, 0.0)
+    point2: tuple = (0.0, 0.0, 0.0)
+    normal: tuple = (0.0, 0.0, 1.0)
+    radius: float = 0.0
+
+    def due(self, t: int) -> bool:
+        """LocalPropertyOutput::ShouldWrite (LocalPropertyOutput.cc:205-210)."""
+        return self.start <= t <= self.stop and t % self.period == 0
+
 
 @dataclass
 class CaseConfig:
@@ -75,6 +136,9 @@ class CaseConfig:
     inlets: list = field(default_factory=list)
     outlets: list = field(default_factory=list)
     units: UnitConverter | None = None
+    t0: int | None = None
+    checkpoint: Path | None = None
+    outputs: list = field(default_factory=list)
 
     def geometry(self) -> Geometry:
         return Geometry.read_gmy(self.geometry_path, str(self.geometry_path).endswith("+"))
@@ -82,9 +146,20 @@ class CaseConfig:
     def params(self, **kw):
         return params(tau=self.tau, **kw)
 
+    def extractions(self, lattice, directory) -> list:
+        """Opens every property output of the case under `directory`."""
+        u = self.units
+        out = []
+        for o in self.outputs:
+            out.append((o, Extraction(lattice, Path(directory) / o.file, [t for t, _ in o.fields],
+                                      selector=o.selector, voxel_size=u.voxel_size, time_step=u.time_step,
+                                      origin=u.origin, point=o.point, normal=o.normal, radius=o.radius,
+                                      point2=o.point2, names=[n for _, n in o.fields])))
+        return out
 
-def _iolet(el, units: UnitConverter):
-    cond = el.find("condition")
+
+def _iolet(el, units: UnitConverter, steps: int, base: Path):
+    cond = _child(el, "condition")
     kind, sub = cond.get("type"), cond.get("subtype")
     position = _vector(_value(el, "position", "lattice"))
     normal = _vector(_value(el, "normal", "dimensionless"))
@@ -96,30 +171,112 @@ def _iolet(el, units: UnitConverter):
         period = units.time(float(_value(cond, "period", "s")))
         return iolet_desc(COSINE, position, normal, density_mean=mean, density_amp=amp, phase=phase,
                           period=period)
+    if kind == "pressure" and sub == "file":
+        # SimConfig.cc:756-768, table by InOutLetFile::CalculateTable
+        path = base / _child(cond, "path").get("value")
+        table, dmin = iolet_file_table(path, False, steps, units.time_step, units.voxel_size)
+        return iolet_desc(FILEPRES, position, normal, table=table, density_min=dmin)
+    if kind == "pressure" and sub in ("WK2", "fileWK"):
+        # SimConfig.cc:770-794, 826-854; fileWK's weights file only feeds
+        # GetScaleFactor, which the path never calls: it runs as WK2
+        r = units.resistance(float(_value(cond, "R", "kg/m^4*s")))
+        c = units.capacitance(float(_value(cond, "C", "m^4*s^2/kg")))
+        radius = units.distance(float(_value(cond, "radius", "m"))) if cond.find("radius") is not None else 0.0
+        area = units.area(float(_value(cond, "area", "m^2")))
+        return iolet_desc(WK2, position, normal, radius=radius, area=area, resistance=r, capacitance=c)
+    if kind == "pressure" and sub == "WK3":
+        # SimConfig.cc:796-824
+        return iolet_desc(WK3, position, normal,
+                          r_c=units.resistance(float(_value(cond, "Rc", "kg/m^4*s"))),
+                          r_p=units.resistance(float(_value(cond, "Rp", "kg/m^4*s"))),
+                          capacitance=units.capacitance(float(_value(cond, "Cp", "m^4*s^2/kg"))),
+                          radius=units.distance(float(_value(cond, "radius", "m"))),
+                          area=units.area(float(_value(cond, "area", "m^2"))))
     if kind == "velocity" and sub == "parabolic":
         radius = float(_value(cond, "radius", "lattice"))
         vmax = units.speed(float(_value(cond, "maximum", "m/s")))
         return iolet_desc(PARABOLIC, position, normal, radius=radius, max_speed=vmax)
+    if kind == "velocity" and sub == "womersley":
+        # SimConfig.cc:878-899
+        return iolet_desc(WOMERSLEY, position, normal, radius=units.distance(float(_value(cond, "radius", "m"))),
+                          pressure_gradient_amplitude=units.pressure_gradient(
+                              float(_value(cond, "pressure_gradient_amplitude", "mmHg/m"))),
+                          period=units.time(float(_value(cond, "period", "s"))),
+                          womersley_number=float(_value(cond, "womersley_number", "dimensionless")))
+    if kind == "velocity" and sub == "file":
+        # SimConfig.cc:937-954, table by InOutLetFileVelocity::CalculateTable
+        path = base / _child(cond, "path").get("value")
+        table, _ = iolet_file_table(path, True, steps, units.time_step, units.voxel_size)
+        return iolet_desc(FILEVEL, position, normal, radius=units.distance(float(_value(cond, "radius", "m"))),
+                          table=table)
     raise ValueError(f"iolet condition {kind}/{sub} is not supported by this reader")
+
+
+def _output(el) -> PropertyOutput:
+    """SimConfig::DoIOForPropertyOutputFile and the selector readers
+    (SimConfig.cc:420-563); points and radii stay in metres."""
+    o = PropertyOutput(file=el.get("file"), period=int(el.get("period")))
+    if el.get("start") is not None:
+        o.start = int(el.get("start"))
+    if el.get("stop") is not None:
+        o.stop = int(el.get("stop"))
+    g = _child(el, "geometry")
+    kind = g.get("type")
+    points = [_vector(p.get("value")) for p in g.findall("point")]
+    radius = g.find("radius")
+    if radius is not None:
+        o.radius = float(radius.get("value"))
+    selectors = {"whole": SELECT_WHOLE, "surface": SELECT_SURFACE, "inlet": SELECT_INLET, "outlet": SELECT_OUTLET,
+                 "plane": SELECT_PLANE, "sphere": SELECT_SPHERE, "line": SELECT_LINE,
+                 "surfacepoint": SELECT_SURFACE_POINT, "surfaceWithinSphere": SELECT_SURFACE_SPHERE}
+    if kind not in selectors:
+        raise ValueError(f"Unrecognised property output geometry selector '{kind}'")
+    o.selector = selectors[kind]
+    if points:
+        o.point = points[0]
+    if kind == "line":
+        o.point2 = points[1]
+    if kind == "plane":
+        o.normal = _vector(_value(g, "normal", "dimensionless"))
+    for f in el.findall("field"):
+        t = f.get("type")
+        if t not in FIELD_TYPES:
+            raise ValueError(f"Unrecognised field type '{t}'")
+        o.fields.append((FIELD_TYPES[t], f.get("name") or t))
+    return o
 
 
 def load_case(xml_path) -> CaseConfig:
     """The collide-and-stream inputs of a HemeLB input.xml."""
     xml_path = Path(xml_path)
+    base = xml_path.parent
     root = ET.parse(xml_path).getroot()
     sim = root.find("simulation")
     dt = float(_value(sim, "step_length", "s"))
     dx = float(_value(sim, "voxel_size", "m"))
     steps = int(float(_value(sim, "steps", "lattice")))
-    units = UnitConverter(dt, dx)
+    origin = _vector(_value(sim, "origin", "m")) if sim.find("origin") is not None else (0.0, 0.0, 0.0)
+    units = UnitConverter(dt, dx, origin)
     tau = 0.5 + (dt * BLOOD_VISCOSITY / BLOOD_DENSITY) / (CS2 * dx * dx)
     gmy = root.find("geometry").find("datafile").get("path")
-    rho0 = 1.0
+    rho0, t0, checkpoint = 1.0, None, None
     ic = root.find("initialconditions")
-    if ic is not None and ic.find("pressure") is not None:
-        uni = ic.find("pressure").find("uniform")
-        if uni is not None:
-            rho0 = units.pressure(float(uni.get("value"))) / CS2
-    inlets = [_iolet(e, units) for e in root.find("inlets").findall("inlet")]
-    outlets = [_iolet(e, units) for e in root.find("outlets").findall("outlet")]
-    return CaseConfig(xml_path.parent / gmy, steps, tau, rho0, inlets, outlets, units)
+    if ic is not None:
+        if ic.find("time") is not None:
+            t0 = int(float(_value(ic, "time", "lattice")))
+        pressure, cp = ic.find("pressure"), ic.find("checkpoint")
+        if pressure is not None and cp is not None:
+            raise ValueError("XML contains both <pressure> and <checkpoint> sub elements of <initialconditions>")
+        if pressure is not None:
+            uni = pressure.find("uniform")
+            if uni is not None:
+                rho0 = units.pressure(float(uni.get("value"))) / CS2
+        elif cp is not None:
+            checkpoint = base / cp.get("file")
+    inlets = [_iolet(e, units, steps, base) for e in root.find("inlets").findall("inlet")]
+    outlets = [_iolet(e, units, steps, base) for e in root.find("outlets").findall("outlet")]
+    outputs = []
+    props = root.find("properties")
+    if props is not None:
+        outputs = [_output(e) for e in props.findall("propertyoutput")]
+    return CaseConfig(base / gmy, steps, tau, rho0, inlets, outlets, units, t0, checkpoint, outputs)

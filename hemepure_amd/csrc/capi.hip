@@ -23,6 +23,7 @@
 #include "d3q19.h"
 #include "errors.h"
 #include "handle.h"
+#include "iolet_math.h"
 #include "lattice.h"
 #include "lbm_kernels.h"
 
@@ -88,6 +89,7 @@ StepArgs make_args(hlb_lattice* h, const double* ain, double* aout) {
   a.bnrm = h->bnrm.p;
   a.G = h->G.p;
   a.ladd_s = h->ladd_s.p;
+  a.ladd_s2 = h->ladd_s.p ? h->ladd_s.p + (size_t)18 * h->SI : nullptr;
   a.rho = h->rho.p;
   a.u = h->u.p;
   a.wss = h->wss.p;
@@ -95,6 +97,7 @@ StepArgs make_args(hlb_lattice* h, const double* ain, double* aout) {
   a.io_n = h->io_n.p;
   a.io_dens = h->io_dens.p;
   a.io_vmax = h->io_vmax.p;
+  a.io_vmax2 = h->io_vmax2.p;
   a.wk_bit = h->wk_bit.p;
   a.wk_contrib = h->wk_contrib.p;
   a.N = h->L.N;
@@ -113,15 +116,54 @@ StepArgs make_args(hlb_lattice* h, const double* ain, double* aout) {
   return a;
 }
 
-// Ghost density (Nash) and peak speed (Ladd) of every iolet at 1-indexed step t.
-int iolet_values(const hlb_lattice* h, int64_t t, double* dens, double* vmax) {
+// The profile factors (s, s2) of a velocity iolet at x: speed = vmax s +
+// vmax2 s2 along the normal.  Parabolic and file profiles: 1 - r^2/a^2
+// (InOutLetParabolicVelocity.cc:30-52, InOutLetFileVelocity.cc:93-109),
+// HLB_ERANGE outside the radius as the reference's exit(16); Womersley:
+// -Re K and Im K, so that vmax s + vmax2 s2 = -Re(K e^{iwt}).
+int velocity_factors(const HostIolet& io, const double x[3], double& s, double& s2) {
+  if (io.d.kind == HLB_IOLET_WOMERSLEY) {
+    double kre, kim;
+    womersley_coefficient(io.d.position, io.n, io.d.radius, io.d.period, io.d.womersley_number,
+                          io.d.pressure_gradient_amplitude, x, &kre, &kim);
+    s = -kre;
+    s2 = kim;
+    return HLB_OK;
+  }
+  const double dx = x[0] - io.d.position[0], dy = x[1] - io.d.position[1], dz = x[2] - io.d.position[2];
+  const double z = dx * io.n[0] + dy * io.n[1] + dz * io.n[2];
+  const double rSq = (dx * dx + dy * dy + dz * dz) - z * z;
+  const double r2a2 = rSq / (io.d.radius * io.d.radius);
+  if (r2a2 > 1.0) {
+    char buf[200];
+    std::snprintf(buf, sizeof buf, "An IOLET site with r = %lf lies outside the IOLET radius %lf.", std::sqrt(rSq),
+                  io.d.radius);
+    return fail(HLB_ERANGE, buf);
+  }
+  s = 1. - r2a2;
+  s2 = 0.0;
+  return HLB_OK;
+}
+
+// Ghost density (Nash) and the speed factors (Ladd, GZS) of every iolet at
+// 1-indexed step t: a velocity iolet's speed along its normal at a link is
+// vmax * s + vmax2 * s2 with the link's profile factors s, s2.
+int iolet_values(const hlb_lattice* h, int64_t t, double* dens, double* vmax, double* vmax2) {
   const int nio = (int)h->iolets.size();
   for (int g = 0; g < nio; ++g) {
     const HostIolet& io = h->iolets[g];
     dens[g] = 1.0;
     vmax[g] = 0.0;
+    vmax2[g] = 0.0;
     switch (io.d.kind) {
       case HLB_IOLET_COSINE: dens[g] = cosine_density(io, (uint64_t)(t - 1)); break;
+      case HLB_IOLET_FILEPRES:
+        // InOutLetFile::GetDensity(Get0IndexedTimeStep()) (InOutLetFile.h:56-59)
+        if (t - 1 >= (int64_t)io.table.size())
+          return fail(HLB_ERANGE, "pressure table of iolet " + std::to_string(g) + " ends before step " +
+                                      std::to_string(t));
+        dens[g] = io.table[t - 1];
+        break;
       case HLB_IOLET_PARABOLIC: {
         // InOutLetParabolicVelocity.cc:47-52
         double max = io.d.max_speed;
@@ -136,6 +178,11 @@ int iolet_values(const hlb_lattice* h, int64_t t, double* dens, double* vmax) {
                                       std::to_string(t));
         vmax[g] = io.table[t];
         break;
+      case HLB_IOLET_WOMERSLEY: {
+        // the time factor exp(i omega t) of InOutLetWomersleyVelocity.cc:35-37
+        womersley_time_factor(io.d.period, t, &vmax[g], &vmax2[g]);
+        break;
+      }
       default: break;
     }
   }
@@ -232,7 +279,7 @@ int ladd_factors(hlb_lattice* h) {
   const HostLattice& L = h->L;
   const int64_t SI = h->SI;
   std::vector<double>& s = h->h_ladd_s;
-  s.assign((size_t)18 * SI, 0.0);
+  s.assign((size_t)36 * SI, 0.0);
   for (int64_t i = 0; i < h->NI; ++i) {
     const int64_t k = h->isite[i];
     const int c = L.collision_class(k);
@@ -242,20 +289,14 @@ int ladd_factors(hlb_lattice* h) {
     const HostIolet& io = h->iolets[side_iolet(h, k)];
     for (int d = 1; d < Q; ++d) {
       if (!((L.iolet[k] >> (d - 1)) & 1u)) continue;
-      const double hx = (double)L.coords[3 * k] + 0.5 * CX[d];
-      const double hy = (double)L.coords[3 * k + 1] + 0.5 * CY[d];
-      const double hz = (double)L.coords[3 * k + 2] + 0.5 * CZ[d];
-      const double dx = hx - io.d.position[0], dy = hy - io.d.position[1], dz = hz - io.d.position[2];
-      const double z = dx * io.n[0] + dy * io.n[1] + dz * io.n[2];
-      const double rSq = (dx * dx + dy * dy + dz * dz) - z * z;
-      const double r2a2 = rSq / (io.d.radius * io.d.radius);
-      if (r2a2 > 1.0) {
-        char buf[200];
-        std::snprintf(buf, sizeof buf, "An IOLET site with r = %lf lies outside the IOLET radius %lf.",
-                      std::sqrt(rSq), io.d.radius);
-        return fail(HLB_ERANGE, buf);
-      }
-      s[(size_t)(d - 1) * SI + i] = 1. - r2a2;
+      // LaddIoletDelegate::StreamLink: the velocity at the link's half-way point
+      const double x[3] = {(double)L.coords[3 * k] + 0.5 * CX[d], (double)L.coords[3 * k + 1] + 0.5 * CY[d],
+                           (double)L.coords[3 * k + 2] + 0.5 * CZ[d]};
+      double f1 = 0.0, f2 = 0.0;
+      const int rc = velocity_factors(io, x, f1, f2);
+      if (rc != HLB_OK) return rc;
+      s[(size_t)(d - 1) * SI + i] = f1;
+      s[(size_t)(18 + d - 1) * SI + i] = f2;
     }
   }
   return HLB_OK;
@@ -334,7 +375,7 @@ int gzs_setup(hlb_lattice* h) {
     int64_t k, nb;
     int d, mode, gi;
     float q;
-    double s;
+    double s, s2;
   };
   std::vector<Pending> pend;
   std::vector<uint8_t> needP(L.N, 0);
@@ -348,26 +389,17 @@ int gzs_setup(hlb_lattice* h) {
       const float q = L.q[k * 18 + d - 1];
       int mode = 1, gi = -1;
       int64_t nb = -1;
-      double sfac = 0.0;
+      double sfac = 0.0, sfac2 = 0.0;
       if ((double)q < 0.75) {
         if ((im >> (i - 1)) & 1u) {
           gi = side_iolet(h, k);
           const HostIolet& io = h->iolets[gi];
-          if (io.d.kind != HLB_IOLET_PARABOLIC && io.d.kind != HLB_IOLET_FILEVEL) continue;  // bounce-back
-          const double x = (double)L.coords[3 * k] + CX[i], y = (double)L.coords[3 * k + 1] + CY[i],
-                       z = (double)L.coords[3 * k + 2] + CZ[i];
-          const double dx = x - io.d.position[0], dy = y - io.d.position[1], dz = z - io.d.position[2];
-          const double zz = dx * io.n[0] + dy * io.n[1] + dz * io.n[2];
-          const double rSq = (dx * dx + dy * dy + dz * dz) - zz * zz;
-          const double r2a2 = rSq / (io.d.radius * io.d.radius);
-          if (r2a2 > 1.0) {
-            char buf[200];
-            std::snprintf(buf, sizeof buf, "An IOLET site with r = %lf lies outside the IOLET radius %lf.",
-                          std::sqrt(rSq), io.d.radius);
-            return fail(HLB_ERANGE, buf);
-          }
+          if (!io.IsVelocity()) continue;  // bounce-back
+          const double x[3] = {(double)L.coords[3 * k] + CX[i], (double)L.coords[3 * k + 1] + CY[i],
+                               (double)L.coords[3 * k + 2] + CZ[i]};
+          const int rc = velocity_factors(io, x, sfac, sfac2);
+          if (rc != HLB_OK) return rc;
           mode = 3;
-          sfac = 1. - r2a2;
         } else if ((wm >> (i - 1)) & 1u) {
           continue;  // bounce-back
         } else {
@@ -379,7 +411,7 @@ int gzs_setup(hlb_lattice* h) {
       }
       needP[k] = 1;
       h->h_bgz[b] |= 1u << (d - 1);
-      pend.push_back({k, nb, d, mode, gi, q, sfac});
+      pend.push_back({k, nb, d, mode, gi, q, sfac, sfac2});
     }
   }
   std::vector<int32_t> pidx(L.N, -1);
@@ -396,7 +428,7 @@ int gzs_setup(hlb_lattice* h) {
     h->h_pmask.push_back(pm);
   }
   for (const Pending& p : pend)
-    h->gz_links_host.push_back(GzsLink{pidx[p.k], p.nb >= 0 ? pidx[p.nb] : -1, p.d, p.mode, p.gi, p.q, p.s});
+    h->gz_links_host.push_back(GzsLink{pidx[p.k], p.nb >= 0 ? pidx[p.nb] : -1, p.d, p.mode, p.gi, p.q, p.s, p.s2});
   h->SP = round_up(std::max<int64_t>(h->NP, 1), 64);
   return HLB_OK;
 }
@@ -428,7 +460,7 @@ int upload_lattice(hlb_lattice* h) {
   HLB_CHECK(h->bii.alloc((size_t)SB));
   HLB_CHECK(h->bq.alloc((size_t)18 * SB));
   HLB_CHECK(h->bnrm.alloc((size_t)3 * SB));
-  HLB_CHECK(h->ladd_s.alloc((size_t)18 * SI));
+  HLB_CHECK(h->ladd_s.alloc((size_t)36 * SI));
   HLB_CHECK(h->rho.alloc((size_t)S));
   HLB_CHECK(h->u.alloc((size_t)3 * S));
   HLB_CHECK(h->wss.alloc((size_t)S));
@@ -524,9 +556,10 @@ int upload_lattice(hlb_lattice* h) {
     HLB_CHECK(h->io_n.alloc(nd.size()));
     HLB_CHECK(h->io_dens.alloc((size_t)kChunk * nio));
     HLB_CHECK(h->io_vmax.alloc((size_t)kChunk * nio));
+    HLB_CHECK(h->io_vmax2.alloc((size_t)kChunk * nio));
     HLB_CHECK(hipMemcpy(h->io_nf.p, nf.data(), sizeof(float) * nf.size(), hipMemcpyHostToDevice));
     HLB_CHECK(hipMemcpy(h->io_n.p, nd.data(), sizeof(double) * nd.size(), hipMemcpyHostToDevice));
-    HLB_CHECK(hipHostMalloc((void**)&h->h_stage, sizeof(double) * 2 * kChunk * nio, hipHostMallocDefault));
+    HLB_CHECK(hipHostMalloc((void**)&h->h_stage, sizeof(double) * 3 * kChunk * nio, hipHostMallocDefault));
     std::vector<uint32_t> bits(nio, 0u);
     for (size_t g = 0; g < h->iolets.size(); ++g)
       if (h->iolets[g].IsWK() && h->iolets[g].dirs0 != 0) bits[g] = 1u << (INV[h->iolets[g].dirs0] - 1);
@@ -604,15 +637,19 @@ int validate_iolets(const hlb_params& p, const hlb_iolet_desc* in, int nin, cons
     if (n > 0 && !v) return fail(HLB_EINVAL, w + ": null iolet array");
     for (int i = 0; i < n; ++i) {
       const int k = v[i].kind;
-      if (k < HLB_IOLET_COSINE || k > HLB_IOLET_WK3) return fail(HLB_EINVAL, w + ": unknown iolet kind");
+      if (k < HLB_IOLET_COSINE || k > HLB_IOLET_WOMERSLEY) return fail(HLB_EINVAL, w + ": unknown iolet kind");
       if ((k == HLB_IOLET_WK2 || k == HLB_IOLET_WK3) && bc != HLB_IOLET_NASH)
         return fail(HLB_EINVAL, w + ": a Windkessel iolet needs the Nash pressure boundary");
-      const bool vel = (k == HLB_IOLET_PARABOLIC || k == HLB_IOLET_FILEVEL);
+      const bool vel = (k == HLB_IOLET_PARABOLIC || k == HLB_IOLET_FILEVEL || k == HLB_IOLET_WOMERSLEY);
       if (bc == HLB_IOLET_LADD && !vel)
         return fail(HLB_EINVAL, w + ": the Ladd boundary needs a velocity iolet (LaddIoletDelegate.h:51-53)");
       if (vel && !(v[i].radius > 0.0)) return fail(HLB_EINVAL, w + ": velocity iolet without radius");
       if (k == HLB_IOLET_FILEVEL && (!v[i].table || v[i].table_len <= 0))
         return fail(HLB_EINVAL, w + ": file velocity iolet without a table");
+      if (k == HLB_IOLET_FILEPRES && (!v[i].table || v[i].table_len <= 0))
+        return fail(HLB_EINVAL, w + ": file pressure iolet without a table");
+      if (k == HLB_IOLET_WOMERSLEY && !(v[i].period != 0.0))
+        return fail(HLB_EINVAL, w + ": Womersley iolet with zero period");
       if (k == HLB_IOLET_COSINE && !(v[i].period != 0.0)) return fail(HLB_EINVAL, w + ": cosine iolet with zero period");
     }
   }
@@ -702,15 +739,20 @@ int configure_iolets(hlb_lattice* h, const hlb_iolet_desc* inlets, int n_inlets,
       io.nf[0] = (float)nx;
       io.nf[1] = (float)ny;
       io.nf[2] = (float)nz;
-      if (v[i].kind == HLB_IOLET_FILEVEL) io.table.assign(v[i].table, v[i].table + v[i].table_len);
+      if (v[i].kind == HLB_IOLET_FILEVEL || v[i].kind == HLB_IOLET_FILEPRES)
+        io.table.assign(v[i].table, v[i].table + v[i].table_len);
       io.d.table = nullptr;
       h->iolets.push_back(std::move(io));
     }
   }
   // Every iolet learns the minimum density any iolet can impose.
   double minDensity = DBL_MAX;
+  // GetDensityMin: cosine mean - amplitude (InOutLetCosine.h:55-58), file the
+  // table's minimum pressure (InOutLetFile.h:48-51), others 1.
   for (const HostIolet& io : h->iolets)
-    minDensity = std::min(minDensity, io.d.kind == HLB_IOLET_COSINE ? io.d.density_mean - io.d.density_amp : 1.0);
+    minDensity = std::min(minDensity, io.d.kind == HLB_IOLET_COSINE   ? io.d.density_mean - io.d.density_amp
+                                      : io.d.kind == HLB_IOLET_FILEPRES ? io.d.density_min
+                                                                        : 1.0);
   for (HostIolet& io : h->iolets) io.minSimDensity = minDensity;
   rc = ladd_factors(h);
   if (rc != HLB_OK) return rc;
@@ -740,6 +782,7 @@ Member32 member_record(const hlb_lattice* h, int props_at) {
   m.bnrm = h->bnrm.p;
   m.G = h->G32.p;
   m.ladd_s = h->ladd_s.p;
+  m.ladd_s2 = h->ladd_s.p ? h->ladd_s.p + (size_t)18 * h->SI : nullptr;
   m.rho = h->rho.p;
   m.u = h->u.p;
   m.wss = h->wss.p;
@@ -747,6 +790,7 @@ Member32 member_record(const hlb_lattice* h, int props_at) {
   m.io_n = h->io_n.p;
   m.io_dens = h->io_dens.p;
   m.io_vmax = h->io_vmax.p;
+  m.io_vmax2 = h->io_vmax2.p;
   m.wk_bit = h->wk_bit.p;
   m.wk_contrib = h->wk_contrib.p;
   m.N = h->L.N;
@@ -794,12 +838,14 @@ int stage_iolet_values(hlb_lattice* h, int64_t t, int64_t n, hipStream_t stream)
   const int nio = std::max<int>(1, (int)h->iolets.size());
   double* dens = h->h_stage;
   double* vmax = h->h_stage + (size_t)kChunk * nio;
+  double* vmax2 = h->h_stage + (size_t)2 * kChunk * nio;
   for (int64_t s = 0; s < n; ++s) {
-    const int rc = iolet_values(h, t + s, dens + s * nio, vmax + s * nio);
+    const int rc = iolet_values(h, t + s, dens + s * nio, vmax + s * nio, vmax2 + s * nio);
     if (rc != HLB_OK) return rc;
   }
   HLB_CHECK(hipMemcpyAsync(h->io_dens.p, dens, sizeof(double) * n * nio, hipMemcpyHostToDevice, stream));
   HLB_CHECK(hipMemcpyAsync(h->io_vmax.p, vmax, sizeof(double) * n * nio, hipMemcpyHostToDevice, stream));
+  HLB_CHECK(hipMemcpyAsync(h->io_vmax2.p, vmax2, sizeof(double) * n * nio, hipMemcpyHostToDevice, stream));
   return HLB_OK;
 }
 
@@ -830,6 +876,7 @@ int enqueue_steps(hlb_lattice* h, int64_t n, bool props_last) {
     StepArgs a = make_args(h, ain, aout);
     a.io_dens = h->io_dens.p + s * nio;
     a.io_vmax = h->io_vmax.p + s * nio;
+    a.io_vmax2 = h->io_vmax2.p + s * nio;
     HLB_CHECK(launch_wk_begin(h->wk_refs.p, nwk, (int)s, h->stream));
     HLB_CHECK(launch_step_fp64(a, h->params.kernel, first, props, h->variant, h->stream));
     HLB_CHECK(launch_gzs_fp64(a, h->params.kernel, h->gz_links.p, (int)h->gz_links_host.size(), a.io_vmax,
@@ -1258,9 +1305,9 @@ int64_t device_bytes(const hlb_lattice* h) {
   int64_t b = 3 * Q * fsz * S + Q * fsz * SI;        // A[2], F, G
   b += 18 * 4 * S + 4 * S;                            // pull, bidx
   b += (4 * 4 + 18 * 4 + 3 * 4) * SB;                 // boundary masks, ids, q, normals
-  b += 18 * 8 * SI + 8 * SI;                          // Ladd factors, Windkessel terms
+  b += 36 * 8 * SI + 8 * SI;                          // Ladd factors, Windkessel terms
   b += 5 * 8 * S;                                     // rho, u, wss
-  b += (int64_t)(2 * kChunk + 8) * 8 * std::max<int64_t>(1, (int64_t)h->iolets.size());  // iolet rows
+  b += (int64_t)(3 * kChunk + 8) * 8 * std::max<int64_t>(1, (int64_t)h->iolets.size());  // iolet rows
   return b;
 }
 
@@ -1284,8 +1331,8 @@ extern "C" int hlb_iolet_state(hlb_handle h, int outlet, int index, double out[4
     out[3] = w.flowRateOld;
     return HLB_OK;
   }
-  std::vector<double> dens(h->iolets.size()), vmax(h->iolets.size());
-  const int rc = iolet_values(h, std::max<int64_t>(1, h->t - 1), dens.data(), vmax.data());
+  std::vector<double> dens(h->iolets.size()), vmax(h->iolets.size()), vmax2(h->iolets.size());
+  const int rc = iolet_values(h, std::max<int64_t>(1, h->t - 1), dens.data(), vmax.data(), vmax2.data());
   if (rc != HLB_OK) return rc;
   out[0] = io.IsWK() ? 1.0 : dens[g];
   out[1] = out[0];

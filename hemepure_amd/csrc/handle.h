@@ -55,6 +55,9 @@ struct HostIolet {
   int64_t centre = -1;         // centre site, reference order (BoundaryValues.cc:109-128)
   bool centreCoupled = false;
   bool IsWK() const { return d.kind == HLB_IOLET_WK2 || d.kind == HLB_IOLET_WK3; }
+  bool IsVelocity() const {
+    return d.kind == HLB_IOLET_PARABOLIC || d.kind == HLB_IOLET_FILEVEL || d.kind == HLB_IOLET_WOMERSLEY;
+  }
 };
 
 template <typename T>
@@ -108,7 +111,7 @@ struct hlb_lattice {
   std::vector<int32_t> bsite;  // reference index of boundary slot b
   std::vector<int32_t> isite;  // reference index of iolet slot i
   int64_t S = 0, NB = 0, SB = 0, NI = 0, SI = 0;
-  hlb::DevBuf<double> A[2], F, G, ladd_s, rho, u, wss, io_n, io_dens, io_vmax, feq_vals;
+  hlb::DevBuf<double> A[2], F, G, ladd_s, rho, u, wss, io_n, io_dens, io_vmax, io_vmax2, feq_vals;
   hlb::DevBuf<uint32_t> bwall, biolet;
   hlb::DevBuf<int32_t> bgi, bii, nbr, slab, bbase;
   hlb::DevBuf<uint64_t> fmask, bmask;
@@ -151,7 +154,7 @@ struct hlb_lattice {
   bool f_valid = false;
   int64_t t = 1;              // 1-indexed step that runs next
   double omega = 0, half_omega = 0, half_omega_minus = 0, les_k = 0;
-  std::vector<double> h_ladd_s;  // Ladd profile factors, 18 planes of SI, device iolet order
+  std::vector<double> h_ladd_s;  // Ladd profile factors, 2 x 18 planes of SI, device iolet order
   bool device_ready = false;
   bool iolets_set = false;   // hlb_create with params, or hlb_set_iolets
   int variant = hlb::kFp32DefaultVariant;  // launch shape of the steady-state step

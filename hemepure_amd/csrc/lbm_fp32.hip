@@ -184,7 +184,7 @@ __device__ __forceinline__ void boundary_fixup32(const Member32& m, const float*
 // shifted form.
 template <int D = 1>
 __device__ __forceinline__ void iolet_links32(const Member32& m, const double* dens, const double* vmax,
-                                              uint32_t i, uint32_t im, int gi, int bc, float rho, float vx,
+                                              const double* vmax2, uint32_t i, uint32_t im, int gi, int bc, float rho, float vx,
                                               float vy, float vz, const float* gpost) {
   if constexpr (D < Q) {
     constexpr int U = INV[D];
@@ -200,7 +200,8 @@ __device__ __forceinline__ void iolet_links32(const Member32& m, const double* d
         const float usq15 = 1.5f * (gx * gx + gy * gy + gz * gz) * inv;
         val = geq_dir<U>(drg, inv, usq15, gx, gy, gz);
       } else {
-        const float s = (float)(ldp(vmax, (uint32_t)gi) * ldp(m.ladd_s + (int64_t)(D - 1) * m.SI, i));
+        const float s = (float)(ldp(vmax, (uint32_t)gi) * ldp(m.ladd_s + (int64_t)(D - 1) * m.SI, i) +
+                                ldp(vmax2, (uint32_t)gi) * ldp(m.ladd_s2 + (int64_t)(D - 1) * m.SI, i));
         const float wx = (float)ldp(m.io_n, 3u * gi) * s * rho, wy = (float)ldp(m.io_n, 3u * gi + 1) * s * rho,
                     wz = (float)ldp(m.io_n, 3u * gi + 2) * s * rho;
         const float correction = 2.0f * wf(D) * cdotf<D>(wx, wy, wz) * 3.0f;
@@ -208,7 +209,7 @@ __device__ __forceinline__ void iolet_links32(const Member32& m, const double* d
       }
       stp(plane(m.G, m.SI, U), i, val);
     }
-    iolet_links32<D + 1>(m, dens, vmax, i, im, gi, bc, rho, vx, vy, vz, gpost);
+    iolet_links32<D + 1>(m, dens, vmax, vmax2, i, im, gi, bc, rho, vx, vy, vz, gpost);
   }
 }
 
@@ -322,7 +323,8 @@ __global__ __launch_bounds__(B) void step_fp32(const Member32* __restrict__ memb
       const int bc = gi < m.n_in ? m.inlet_bc : m.outlet_bc;
       const double* dens = m.io_dens + (int64_t)step * m.nio;
       const double* vmax = m.io_vmax + (int64_t)step * m.nio;
-      iolet_links32(m, dens, vmax, (uint32_t)ii, im, gi, bc, rho, ux, uy, uz, gpost);
+      const double* vmax2 = m.io_vmax2 + (int64_t)step * m.nio;
+      iolet_links32(m, dens, vmax, vmax2, (uint32_t)ii, im, gi, bc, rho, ux, uy, uz, gpost);
       if (im & ldp(m.wk_bit, (uint32_t)gi)) {
         const double c = (double)ux * -ldp(m.io_n, 3u * gi) + (double)uy * -ldp(m.io_n, 3u * gi + 1) +
                          (double)uz * -ldp(m.io_n, 3u * gi + 2);
@@ -379,7 +381,8 @@ __global__ __launch_bounds__(kBlock) void gzs_links_fp32(const Member32* __restr
     for (int j = 0; j < Q; ++j) gneq[j] = q * gneq[j] + (1.0f - q) * (h[j] - heq[j]);
   } else if (lk.mode == 3) {
     const double* vmax = m.io_vmax + (int64_t)step * m.nio;
-    const float sp = (float)(ldp(vmax, (uint32_t)lk.gi) * lk.s);
+    const double* vmax2 = m.io_vmax2 + (int64_t)step * m.nio;
+    const float sp = (float)(ldp(vmax, (uint32_t)lk.gi) * lk.s + ldp(vmax2, (uint32_t)lk.gi) * lk.s2);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const float v = (float)ldp(m.io_n, 3u * lk.gi + c) * sp;

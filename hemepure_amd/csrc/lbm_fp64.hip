@@ -130,7 +130,9 @@ __device__ __forceinline__ void iolet_links(const StepArgs& a, uint32_t i, uint3
         const FeqPre p = feq_pre(rg, gx, gy, gz);
         val = feq_dir<U>(p, gx, gy, gz);
       } else {
-        const double m = a.io_vmax[gi] * ldp(a.ladd_s + (int64_t)(D - 1) * a.SI, i);
+        // speed along the normal: vmax s (parabolic, file) or -Re(K e^{iwt}) (Womersley)
+        const double m = a.io_vmax[gi] * ldp(a.ladd_s + (int64_t)(D - 1) * a.SI, i) +
+                         a.io_vmax2[gi] * ldp(a.ladd_s2 + (int64_t)(D - 1) * a.SI, i);
         const double wx = a.io_n[3 * gi] * m * rho, wy = a.io_n[3 * gi + 1] * m * rho,
                      wz = a.io_n[3 * gi + 2] * m * rho;
         const double correction =
@@ -438,7 +440,7 @@ __global__ __launch_bounds__(kBlock) void gzs_links_fp64(StepArgs a, const GzsLi
 #pragma unroll
     for (int j = 0; j < Q; ++j) fneq[j] = q * fneq[j] + (1. - q) * (g[j] - gfeq[j]);
   } else if (lk.mode == 3) {
-    const double sp = ldp(vmax, (uint32_t)lk.gi) * lk.s;
+    const double sp = ldp(vmax, (uint32_t)lk.gi) * lk.s + ldp(a.io_vmax2, (uint32_t)lk.gi) * lk.s2;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const double v = ldp(a.io_n, 3u * lk.gi + c) * sp;

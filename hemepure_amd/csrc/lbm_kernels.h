@@ -84,7 +84,7 @@ struct GzsLink {
   int32_t mode;     // 1: this site only, 2: with the next site out, 3: velocity iolet beyond
   int32_t gi;       // global iolet index (mode 3)
   float q;          // wall distance as stored
-  double s;         // 1 - r^2/a^2 of the iolet profile at x + c_INV[d] (mode 3)
+  double s, s2;     // iolet profile factors at x + c_INV[d] (mode 3): speed = vmax s + vmax2 s2
 };
 
 struct StepArgs {
@@ -100,14 +100,16 @@ struct StepArgs {
   const float* __restrict__ bq;
   const float* __restrict__ bnrm;
   double* __restrict__ G;
-  const double* __restrict__ ladd_s;
+  const double* __restrict__ ladd_s;   // 18 planes of SI: velocity profile factor per iolet link
+  const double* __restrict__ ladd_s2;  // 18 planes of SI: second factor (Womersley), else 0
   double* __restrict__ rho;
   double* __restrict__ u;
   double* __restrict__ wss;
   const float* __restrict__ io_nf;    // 3 per iolet: the normal narrowed to float (Nash)
   const double* __restrict__ io_n;    // 3 per iolet: the normal in double (Ladd)
   const double* __restrict__ io_dens; // this step's ghost density per iolet
-  const double* __restrict__ io_vmax; // this step's peak speed per iolet
+  const double* __restrict__ io_vmax; // this step's peak speed per iolet (Womersley: cos wt)
+  const double* __restrict__ io_vmax2; // this step's second speed factor (Womersley: sin wt), else 0
   const uint32_t* __restrict__ wk_bit; // per iolet: the link bit of its Windkessel coupling direction
   double* __restrict__ wk_contrib;     // per iolet site: u.(-n) of this step (coupling sites only)
   int64_t N, S, SB, SI;
@@ -142,6 +144,7 @@ struct Member32 {
   const float* bnrm;
   float* G;               // iolet values for the next step, 19 planes of SI
   const double* ladd_s;
+  const double* ladd_s2;
   double* rho;
   double* u;
   double* wss;
@@ -149,6 +152,7 @@ struct Member32 {
   const double* io_n;
   const double* io_dens;  // rows of nio per step of the launch sequence
   const double* io_vmax;
+  const double* io_vmax2;
   const uint32_t* wk_bit;
   double* wk_contrib;
   int64_t N, S, SB, SI;

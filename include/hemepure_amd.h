@@ -57,6 +57,8 @@ extern "C" {
 #define HLB_IOLET_FILEVEL 2   /* InOutLetFileVelocity, table already in lattice units */
 #define HLB_IOLET_WK2 3       /* InOutLetWK2 */
 #define HLB_IOLET_WK3 4       /* InOutLetWK3 */
+#define HLB_IOLET_FILEPRES 5  /* InOutLetFile: density per 0-indexed step (hlb_iolet_file_table) */
+#define HLB_IOLET_WOMERSLEY 6 /* InOutLetWomersleyVelocity */
 
 /* Storage precision of the distributions. */
 #define HLB_FP64 0
@@ -85,7 +87,7 @@ typedef struct hlb_geometry {
 /* One inlet or outlet: position/normal/radius from the XML plus the
  * provider's parameters (src/configuration/SimConfig.cc:268-400, 725-830). */
 typedef struct hlb_iolet_desc {
-  int32_t kind;            /* HLB_IOLET_COSINE .. HLB_IOLET_WK3 */
+  int32_t kind;            /* HLB_IOLET_COSINE .. HLB_IOLET_WOMERSLEY */
   int32_t warmup;          /* cosine / parabolic warm-up steps */
   double position[3];      /* lattice units */
   double normal[3];        /* normalised in double, as InOutLet::SetNormal */
@@ -93,10 +95,14 @@ typedef struct hlb_iolet_desc {
   double density_mean;     /* cosine */
   double density_amp;      /* cosine */
   double phase;            /* cosine, radians */
-  double period;           /* cosine, lattice steps */
+  double period;           /* cosine, Womersley: lattice steps */
   double max_speed;        /* parabolic */
-  const double* table;     /* file velocity: max speed per 1-indexed step */
+  const double* table;     /* file velocity: max speed per 1-indexed step;
+                              file pressure: density per 0-indexed step */
   int64_t table_len;
+  double density_min;      /* file pressure: GetDensityMin */
+  double pressure_gradient_amplitude; /* Womersley, lattice units */
+  double womersley_number;            /* Womersley */
   double area;             /* Windkessel */
   double resistance;       /* WK2 */
   double r_c;              /* WK3 characteristic resistance */
@@ -408,6 +414,18 @@ int hlb_site_masks(hlb_handle h, uint32_t* wall, uint32_t* iolet, int32_t* iolet
                    int32_t* site_type);
 /* GetCutDistance (n*18, double) and GetNormalToWall (n*3, double). */
 int hlb_wall_geometry(hlb_handle h, double* dist, double* normal);
+
+/* ---- iolet tables from files ------------------------------------------ */
+/* InOutLetFile::CalculateTable (src/lb/iolets/InOutLetFile.cc:37-101) and
+ * InOutLetFileVelocity::CalculateTable (InOutLetFileVelocity.cc:31-90): reads
+ * "time value" pairs (s; mmHg or m/s), clips them to total_steps *
+ * time_step with an interpolated end point, and fills out[0..total_steps]
+ * with the lattice density (pressure) or lattice speed (velocity) of each
+ * step, the profile repeating as the reference repeats it.  out_len must be
+ * at least total_steps + 1.  For pressure, *density_min receives
+ * GetDensityMin (may be NULL). */
+int hlb_iolet_file_table(const char* path, int velocity, int64_t total_steps, double time_step, double voxel_size,
+                         double* out, int64_t out_len, double* density_min);
 
 /* ---- procedural geometries (the inputs a .gmy file would hold) ---------- */
 typedef struct hlb_geom hlb_geom;

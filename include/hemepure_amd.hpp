@@ -238,6 +238,7 @@ inline hlb_iolet_desc Base(int32_t kind, const double position[3], const double 
   d.density_mean = 1.0;
   d.period = 1.0;
   d.area = d.resistance = d.r_c = d.r_p = d.capacitance = 1.0;
+  d.density_min = 1.0;
   return d;
 }
 inline hlb_iolet_desc InOutLetCosine(const double position[3], const double normal[3], LatticeDensity mean,
@@ -268,6 +269,41 @@ inline hlb_iolet_desc InOutLetFileVelocity(const double position[3], const doubl
   d.table_len = (int64_t)table.size();
   return d;
 }
+/* InOutLetWomersleyVelocity (lattice units: radius, pressure-gradient
+ * amplitude, period in steps). */
+inline hlb_iolet_desc InOutLetWomersleyVelocity(const double position[3], const double normal[3], double radius,
+                                                double pressureGradientAmplitude, double period,
+                                                double womersleyNumber) {
+  hlb_iolet_desc d = Base(HLB_IOLET_WOMERSLEY, position, normal);
+  d.radius = radius;
+  d.pressure_gradient_amplitude = pressureGradientAmplitude;
+  d.period = period;
+  d.womersley_number = womersleyNumber;
+  return d;
+}
+/* InOutLetFile: CalculateTable from the "time mmHg" file for a run of
+ * totalSteps steps of timeStep seconds on voxels of voxelSize metres; the
+ * table (density per 0-indexed step) lives in `table`, which must outlive the
+ * LBM's construction. */
+inline hlb_iolet_desc InOutLetFile(const double position[3], const double normal[3], const std::string& path,
+                                   int64_t totalSteps, double timeStep, double voxelSize,
+                                   std::vector<double>& table) {
+  hlb_iolet_desc d = Base(HLB_IOLET_FILEPRES, position, normal);
+  table.assign((size_t)totalSteps + 1, 0.0);
+  detail::Check(hlb_iolet_file_table(path.c_str(), 0, totalSteps, timeStep, voxelSize, table.data(),
+                                     (int64_t)table.size(), &d.density_min));
+  d.table = table.data();
+  d.table_len = (int64_t)table.size();
+  return d;
+}
+/* InOutLetFileVelocity::CalculateTable from a "time m/s" file. */
+inline std::vector<double> FileVelocityTable(const std::string& path, int64_t totalSteps, double timeStep,
+                                             double voxelSize) {
+  std::vector<double> table((size_t)totalSteps + 1);
+  detail::Check(hlb_iolet_file_table(path.c_str(), 1, totalSteps, timeStep, voxelSize, table.data(),
+                                     (int64_t)table.size(), nullptr));
+  return table;
+}
 inline hlb_iolet_desc InOutLetWK2(const double position[3], const double normal[3], double radius, double area,
                                   double resistance, double capacitance) {
   hlb_iolet_desc d = Base(HLB_IOLET_WK2, position, normal);
@@ -276,6 +312,12 @@ inline hlb_iolet_desc InOutLetWK2(const double position[3], const double normal[
   d.resistance = resistance;
   d.capacitance = capacitance;
   return d;
+}
+/* InOutLetFileWK: a WK2 whose weights file only feeds GetScaleFactor, which
+ * the step never calls (InOutLetFileWK.cc:36-124). */
+inline hlb_iolet_desc InOutLetFileWK(const double position[3], const double normal[3], double area,
+                                     double resistance, double capacitance) {
+  return InOutLetWK2(position, normal, 0.0, area, resistance, capacitance);
 }
 inline hlb_iolet_desc InOutLetWK3(const double position[3], const double normal[3], double radius, double area,
                                   double rc, double rp, double capacitance) {

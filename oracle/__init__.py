@@ -33,6 +33,9 @@ class OrcIolet(C.Structure):
         ("max_speed", C.c_double),
         ("table", C.POINTER(C.c_double)),
         ("table_len", C.c_int64),
+        ("density_min", C.c_double),
+        ("pressure_gradient_amplitude", C.c_double),
+        ("womersley_number", C.c_double),
         ("area", C.c_double),
         ("resistance", C.c_double),
         ("r_c", C.c_double),
@@ -98,6 +101,8 @@ def lib() -> C.CDLL:
             "orc_cosine_density": (C.c_double, [C.POINTER(OrcIolet), C.c_double, C.c_uint64]),
             "orc_stress_site": (None, [C.c_int, C.c_double, C.c_double, vp, C.c_int, vp, C.c_int, vp]),
             "orc_get_stress": (None, [vp, vp]),
+            "orc_iolet_velocity": (C.c_int, [C.POINTER(OrcIolet), vp, C.c_uint64, vp]),
+            "orc_bessel_j0": (None, [C.c_double, C.c_double, vp]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -113,7 +118,7 @@ def _p(a: np.ndarray) -> C.c_void_p:
 
 def iolet(kind, position, normal, radius=0.0, density_mean=1.0, density_amp=0.0, phase=0.0, period=1.0,
           warmup=0, max_speed=0.0, table=None, area=1.0, resistance=1.0, r_c=1.0, r_p=1.0,
-          capacitance=1.0) -> OrcIolet:
+          capacitance=1.0, density_min=1.0, pressure_gradient_amplitude=0.0, womersley_number=0.0) -> OrcIolet:
     d = OrcIolet()
     d.kind, d.warmup = kind, warmup
     d.position[:] = tuple(position)
@@ -126,6 +131,8 @@ def iolet(kind, position, normal, radius=0.0, density_mean=1.0, density_amp=0.0,
         d.table = t.ctypes.data_as(C.POINTER(C.c_double))
         d.table_len = t.size
     d.area, d.resistance, d.r_c, d.r_p, d.capacitance = area, resistance, r_c, r_p, capacitance
+    d.density_min, d.pressure_gradient_amplitude = density_min, pressure_gradient_amplitude
+    d.womersley_number = womersley_number
     return d
 
 
@@ -288,6 +295,20 @@ def stress_site(kernel, tau, lam, f, n, wall=True, sse3=True):
     out = np.zeros(17)
     lib().orc_stress_site(kernel, tau, lam, _p(f), 1 if sse3 else 0, _p(n), 1 if wall else 0, _p(out))
     return out
+
+
+def iolet_velocity(io: OrcIolet, x, t: int):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    v = np.zeros(3)
+    if lib().orc_iolet_velocity(C.byref(io), _p(x), int(t), _p(v)) != 0:
+        raise ValueError("site outside the iolet radius")
+    return v
+
+
+def bessel_j0(z: complex) -> complex:
+    out = np.zeros(2)
+    lib().orc_bessel_j0(z.real, z.imag, _p(out))
+    return complex(out[0], out[1])
 
 
 def cosine_density(io: OrcIolet, min_density: float, t0: int) -> float:

@@ -24,6 +24,7 @@
 #include <array>
 #include <cfloat>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstring>
 #include <limits>
@@ -500,9 +501,12 @@ struct Iolet {
   double flowRateOld = 0.0, flowRate = 0.0, flowRateNew = 0.0;
   int64_t siteCount = 0;
   bool IsWK() const { return d.kind == ORC_IOLET_WK2 || d.kind == ORC_IOLET_WK3; }
-  bool IsVelocity() const { return d.kind == ORC_IOLET_PARABOLIC || d.kind == ORC_IOLET_FILEVEL; }
+  bool IsVelocity() const {
+    return d.kind == ORC_IOLET_PARABOLIC || d.kind == ORC_IOLET_FILEVEL || d.kind == ORC_IOLET_WOMERSLEY;
+  }
   double DensityMin() const {
     if (d.kind == ORC_IOLET_COSINE) return d.density_mean - d.density_amp;  // InOutLetCosine.h:46-49
+    if (d.kind == ORC_IOLET_FILEPRES) return d.density_min;                 // InOutLetFile.h:48-51
     return 1.0;  // velocity and Windkessel iolets
   }
 };
@@ -521,15 +525,56 @@ double CosineDensity(const orc_iolet& d, double minSimDensity, uint64_t time_ste
 double BoundaryDensity(const Iolet& io, uint64_t t0) {
   switch (io.d.kind) {
     case ORC_IOLET_COSINE: return CosineDensity(io.d, io.minSimDensity, t0);
+    // InOutLetFile::GetDensity (InOutLetFile.h:56-59); past the table: NaN
+    case ORC_IOLET_FILEPRES: return t0 < io.table.size() ? io.table[t0] : std::nan("");
     case ORC_IOLET_WK2:
     case ORC_IOLET_WK3: return io.density;
     default: return 1.0;
   }
 }
 
+// util::BesselJ0ComplexArgument, src/util/Bessel.cc:18-45 (tolSq 1e-12, Bessel.h:36).
+std::complex<double> BesselJ0ComplexArgument(const std::complex<double>& z, double tolSq = 1e-12) {
+  std::complex<double> sum(1.0, 0.0);
+  double fact = 1;
+  std::complex<double> zSqOver4_pow(1.0, 0.0);
+  std::complex<double> term(1.0, 0.0);
+  for (unsigned i = 1; term.real() * term.real() + term.imag() * term.imag() > tolSq; ++i) {
+    fact *= i;
+    zSqOver4_pow *= -0.25 * z * z;
+    term = zSqOver4_pow / (fact * fact);
+    sum += term;
+  }
+  return sum;
+}
+
+// InOutLetWomersleyVelocity::GetVelocity, src/lb/iolets/InOutLetWomersleyVelocity.cc:24-41.
+void WomersleyVelocity(const Iolet& io, const double x[3], uint64_t t, double v[3]) {
+  typedef std::complex<double> Complex;
+  const Complex i(0, 1);
+  const Complex iPowThreeHalves = pow(i, 1.5);
+  const double displ[3] = {x[0] - io.d.position[0], x[1] - io.d.position[1], x[2] - io.d.position[2]};
+  const double z = displ[0] * io.normal[0] + displ[1] * io.normal[1] + displ[2] * io.normal[2];
+  const double r = std::sqrt((displ[0] * displ[0] + displ[1] * displ[1] + displ[2] * displ[2]) - z * z);
+  const double omega = 2.0 * PI / io.d.period;
+  const double density = 1.0;
+  const Complex besselNumer = BesselJ0ComplexArgument(iPowThreeHalves * io.d.womersley_number * r / io.d.radius);
+  const Complex besselDenom = BesselJ0ComplexArgument(iPowThreeHalves * io.d.womersley_number);
+  const double velocityMagnitude = std::real(io.d.pressure_gradient_amplitude / (density * omega) *
+                                             (1.0 - besselNumer / besselDenom) * exp(i * omega * double(t)));
+  v[0] = io.normal[0] * -velocityMagnitude;
+  v[1] = io.normal[1] * -velocityMagnitude;
+  v[2] = io.normal[2] * -velocityMagnitude;
+}
+
 // Parabolic / file velocity at x, src/lb/iolets/InOutLetParabolicVelocity.cc:28-58
-// and InOutLetFileVelocity.cc:85-101 (table branch).  t is the 1-indexed step.
+// and InOutLetFileVelocity.cc:85-101 (table branch); Womersley as above.  t is
+// the 1-indexed step.
 bool IoletVelocity(const Iolet& io, const double x[3], uint64_t t, double v[3], std::string& msg) {
+  if (io.d.kind == ORC_IOLET_WOMERSLEY) {
+    WomersleyVelocity(io, x, t, v);
+    return true;
+  }
   const double dx = x[0] - io.d.position[0], dy = x[1] - io.d.position[1], dz = x[2] - io.d.position[2];
   const double z = dx * io.normal[0] + dy * io.normal[1] + dz * io.normal[2];
   const double rSq = (dx * dx + dy * dy + dz * dz) - z * z;
@@ -648,7 +693,7 @@ extern "C" orc_lbm* orc_lbm_create(orc_lattice* l, const orc_params* p, const or
       io.normal[0] = nx;
       io.normal[1] = ny;
       io.normal[2] = nz;
-      if (src[i].kind == ORC_IOLET_FILEVEL && src[i].table)
+      if ((src[i].kind == ORC_IOLET_FILEVEL || src[i].kind == ORC_IOLET_FILEPRES) && src[i].table)
         io.table.assign(src[i].table, src[i].table + src[i].table_len);
       io.dirs0 = SortDirections(io.normal);
       // Centre site: first site within +-1 of the position in every axis
@@ -998,6 +1043,23 @@ extern "C" void orc_collide(int kernel, double tau, double lambda, const double*
 extern "C" double orc_wss(double rho, double tau, const double* fpost, const double* f, const double n[3]) {
   (void)rho;
   return WallShearStress(tau, fpost, f, n);
+}
+extern "C" int orc_iolet_velocity(const orc_iolet* d, const double x[3], uint64_t t, double v[3]) {
+  Iolet io;
+  io.d = *d;
+  double nx = d->normal[0], ny = d->normal[1], nz = d->normal[2];
+  const double mag = std::sqrt(nx * nx + ny * ny + nz * nz);
+  io.normal[0] = nx / mag;
+  io.normal[1] = ny / mag;
+  io.normal[2] = nz / mag;
+  if (d->table) io.table.assign(d->table, d->table + d->table_len);
+  std::string msg;
+  return IoletVelocity(io, x, t, v, msg) ? 0 : -1;
+}
+extern "C" void orc_bessel_j0(double re, double im, double out[2]) {
+  const std::complex<double> j = BesselJ0ComplexArgument(std::complex<double>(re, im));
+  out[0] = j.real();
+  out[1] = j.imag();
 }
 extern "C" void orc_stress_site(int kernel, double tau, double lambda, const double* f, int sse3, const double n[3],
                                 int wall, double* out) {

@@ -48,6 +48,8 @@ extern "C" {
 #define ORC_IOLET_FILEVEL 2    /* InOutLetFileVelocity with a given lattice-unit table */
 #define ORC_IOLET_WK2 3        /* InOutLetWK2 */
 #define ORC_IOLET_WK3 4        /* InOutLetWK3 */
+#define ORC_IOLET_FILEPRES 5   /* InOutLetFile with a given density table (0-indexed steps) */
+#define ORC_IOLET_WOMERSLEY 6  /* InOutLetWomersleyVelocity */
 
 typedef struct orc_iolet {
   int32_t kind;
@@ -60,8 +62,12 @@ typedef struct orc_iolet {
   double phase;            /* cosine */
   double period;           /* cosine, lattice steps */
   double max_speed;        /* parabolic */
-  const double* table;     /* file velocity: max speed per 1-indexed step */
+  const double* table;     /* file velocity: max speed per 1-indexed step;
+                              file pressure: density per 0-indexed step */
   int64_t table_len;
+  double density_min;      /* file pressure: GetDensityMin */
+  double pressure_gradient_amplitude; /* Womersley, lattice units */
+  double womersley_number;            /* Womersley */
   double area;             /* Windkessel */
   double resistance;       /* WK2 R */
   double r_c;              /* WK3 characteristic resistance */
@@ -134,6 +140,10 @@ double orc_cosine_density(const orc_iolet* io, double min_density, uint64_t t0);
  * normal projections; the last nine zero unless wall). */
 void orc_stress_site(int kernel, double tau, double lambda, const double* f, int sse3, const double n[3],
                      int wall, double* out);
+/* Velocity of a velocity iolet at x and 1-indexed step t (0, or -1 outside
+ * a parabolic radius), and util::BesselJ0ComplexArgument. */
+int orc_iolet_velocity(const orc_iolet* io, const double x[3], uint64_t t, double v[3]);
+void orc_bessel_j0(double re, double im, double out[2]);
 /* The same for every site from the state entering the next step, n*17. */
 void orc_get_stress(const orc_lbm* m, double* out);
 

@@ -116,13 +116,18 @@ class IoletSpec:
     r_c: float = 1.0
     r_p: float = 1.0
     capacitance: float = 1.0
+    density_min: float = 1.0
+    pressure_gradient_amplitude: float = 0.0
+    womersley_number: float = 0.0
 
     def kw(self):
         return dict(position=self.position, normal=self.normal, radius=self.radius,
                     density_mean=self.density_mean, density_amp=self.density_amp, phase=self.phase,
                     period=self.period, warmup=self.warmup, max_speed=self.max_speed, table=self.table,
                     area=self.area, resistance=self.resistance, r_c=self.r_c, r_p=self.r_p,
-                    capacitance=self.capacitance)
+                    capacitance=self.capacitance, density_min=self.density_min,
+                    pressure_gradient_amplitude=self.pressure_gradient_amplitude,
+                    womersley_number=self.womersley_number)
 
 
 @dataclass

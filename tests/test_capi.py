@@ -31,7 +31,9 @@ def test_struct_layouts_match_header_sizes():
     # 4*int32 + int64 + 6 pointers
     assert ctypes.sizeof(H.GeometryView) == 16 + 8 + 6 * 8
     assert ctypes.sizeof(H.Params) == 6 * 4 + 2 * 8
-    assert ctypes.sizeof(H.IoletDesc) == 8 + 6 * 8 + 6 * 8 + 8 + 8 + 5 * 8
+    # kind, warmup; position, normal; radius..max_speed; table, table_len; density_min,
+    # pressure_gradient_amplitude, womersley_number; area..capacitance
+    assert ctypes.sizeof(H.IoletDesc) == 8 + 6 * 8 + 6 * 8 + 8 + 8 + 3 * 8 + 5 * 8
 
 
 def _cyl():

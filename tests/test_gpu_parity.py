@@ -16,7 +16,7 @@ from fixtures import (Case, IoletSpec, anisotropic_f, bifurcation_case, closed_b
 pytestmark = pytest.mark.gpu
 
 
-def assert_parity(name, got, want, rel=1e-12):
+def assert_parity(name, got, want, rel=1e-12, exact=True):
     got = np.asarray(got)
     want = np.asarray(want)
     assert got.shape == want.shape, name
@@ -26,10 +26,14 @@ def assert_parity(name, got, want, rel=1e-12):
     finite = np.isfinite(w) & (np.abs(w) < 1e300)
     scale = np.abs(w[finite]).max() if finite.any() else 1.0
     err = np.abs(g - w)
-    bound = rel * np.abs(w) + 1e-3 * rel * scale
+    # exact: elementwise; otherwise relative to the field's scale (components
+    # near zero, like the transverse velocity of an axial flow, carry the
+    # absolute error of the field)
+    bound = rel * np.abs(w) + 1e-3 * rel * scale if exact else rel * scale
     bad = ~(err <= bound) & ~(g == w)
     assert not bad.any(), f"{name}: {bad.sum()} values beyond {rel} relative, max abs err {err[bad].max()}"
-    assert np.array_equal(g, w), f"{name}: within {rel} but not bit-identical ({int((g != w).sum())} values)"
+    if exact:
+        assert np.array_equal(g, w), f"{name}: within {rel} but not bit-identical ({int((g != w).sum())} values)"
 
 
 def run_both(case: Case, steps: int, f0=None, t0=1, properties=True):
@@ -46,13 +50,16 @@ def run_both(case: Case, steps: int, f0=None, t0=1, properties=True):
     return lat, ref
 
 
-def compare_all(lat, ref, tag):
-    assert_parity(f"{tag} f", lat.download_f(), ref.f())
+def compare_all(lat, ref, tag, exact=True):
+    assert_parity(f"{tag} f", lat.download_f(), ref.f(), exact=exact)
     rho, u, wss = lat.fields()
     orho, ou, owss = ref.fields()
-    assert_parity(f"{tag} rho", rho, orho)
-    assert_parity(f"{tag} u", u, ou)
-    assert_parity(f"{tag} wss", wss, owss)
+    assert_parity(f"{tag} rho", rho, orho, exact=exact)
+    assert_parity(f"{tag} u", u, ou, exact=exact)
+    # WSS = sqrt(|s|^2 - (s.n)^2) (Lattice.h:1122-1156) loses digits where the
+    # shear part is small against the normal stress: a 1e-12 state difference
+    # moves it by up to ~1e-12 |s|^2 / WSS, so the inexact bar is looser there.
+    assert_parity(f"{tag} wss", wss, owss, rel=1e-12 if exact else 1e-9, exact=exact)
 
 
 def test_cylinder_lbgk_sbb_nash_2000_steps():

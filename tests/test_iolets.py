@@ -1,0 +1,206 @@
+"""Iolet value providers on the CPU: the file tables (product code, pinned by
+the reference's own iolet tests and data files) and the oracle's Womersley
+profile (pinned by the reference's Womersley test and by an independent Bessel
+function).
+
+Expectations: src/unittests/lbtests/iolets/InOutLetTests.h --
+TestFileConstruct (:117-166), TestWomersleyVelocityConstruct (:200-275),
+TestFileVelocityConstruct (:277-333).
+"""
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+import scipy.special
+
+import hemepure_amd as H
+import oracle as O
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+MMHG = 133.3223874
+CS2 = 1.0 / 3.0
+
+
+def test_file_pressure_table_matches_reference_test():
+    """config_file_inlet.xml: 40000 steps of 1e-4 s, voxel 0.01 m; iolet.txt
+    rises 78 -> 82 -> 78 mmHg over 4 s.  GetDensity(0) and GetDensityMin are
+    the density of 78 mmHg, GetDensity(total / 2) that of 82 mmHg, 1e-6."""
+    total, dt, dx = 40000, 1e-4, 0.01
+    table, dmin = H.iolet_file_table(GOLDEN / "iolet_pressure.txt", False, total, dt, dx)
+    assert table.shape == (total + 1,)
+    temp = dt / dx
+
+    def target(p):
+        return 1 + p * MMHG * temp * temp / (CS2 * 1000.0)
+
+    assert abs(table[0] - target(78.0)) < 1e-6
+    assert abs(dmin - target(78.0)) < 1e-6
+    assert abs(table[total // 2] - target(82.0)) < 1e-6
+    # piecewise linear in between: a quarter of the way, 80 mmHg
+    assert abs(table[total // 4] - target(80.0)) < 1e-6
+
+
+def test_file_velocity_table_matches_reference_test():
+    """config_file_velocity_inlet.xml: 4000 steps of 1e-3 s, voxel 0.01 m;
+    the centreline speed at t = 3 s is 0.01 m/s (InOutLetTests.h:307-316)."""
+    total, dt, dx = 4000, 1e-3, 0.01
+    table, _ = H.iolet_file_table(GOLDEN / "iolet_velocity.txt", True, total, dt, dx)
+    speed = dx / dt
+    assert abs(table[3000] * speed - 0.01) < 1e-9
+    assert abs(table[1500] * speed - 0.025) < 1e-9  # halfway between 0.02 and 0.03
+    # the Ladd profile at half the radius is 3/4 of the centreline (0.0075 m/s)
+    io = O.iolet(H.FILEVEL, (0.0, 0.0, -5.0), (0.0, 0.0, 1.0), radius=20.0, table=table)
+    v = O.iolet_velocity(io, (-10.0, 0.0, -5.0), 3000)
+    assert abs(v[2] * speed - 0.0075) < 1e-9 and abs(v[0]) < 1e-15
+
+
+def test_file_table_clips_and_repeats(tmp_path):
+    """A profile longer than the run is cut at the run's end with an
+    interpolated end point; a shorter one repeats."""
+    f = tmp_path / "p.txt"
+    f.write_text("0.0 0.0\n1.0 10.0\n3.0 30.0\n")
+    table, dmin = H.iolet_file_table(f, True, 20, 0.1, 1.0)  # run ends at 2 s
+    # value 10 t m/s, lattice speed 10 m/s: t in lattice units; the last step
+    # wraps to the profile's start (timeStep % steps-in-profile, as the reference)
+    np.testing.assert_allclose(table[:20], np.arange(20) * 0.1, rtol=1e-12)
+    assert table[20] == table[0]
+    g = tmp_path / "q.txt"
+    g.write_text("0.0 0.0\n1.0 10.0\n")
+    table, _ = H.iolet_file_table(g, True, 30, 0.1, 1.0)  # 1 s profile, 3 s run
+    assert table[0] == table[10] == table[20] == 0.0
+    with pytest.raises(H.HlbError):
+        H.iolet_file_table(tmp_path / "missing.txt", True, 10, 0.1, 1.0)
+    out = np.zeros(5)
+    rc = H.lib().hlb_iolet_file_table(str(f).encode(), 1, 10, 0.1, 1.0, H._ptr(out), out.size, None)
+    assert rc < 0 and "total_steps" in H.lib().hlb_last_error().decode()
+
+
+@pytest.mark.parametrize("z", [0.5 + 0j, 2.0 * complex(-math.sqrt(0.5), math.sqrt(0.5)),
+                               1e-4 * complex(-math.sqrt(0.5), math.sqrt(0.5)), 3.0 - 1.5j])
+def test_oracle_bessel_j0_matches_scipy(z):
+    got = O.bessel_j0(z)
+    want = complex(scipy.special.jv(0, z))
+    assert abs(got - want) <= 1e-6 * max(1.0, abs(want))
+
+
+def womersley(alpha, period, radius=10.0, pga=MMHG * 1e-6):
+    return O.iolet(H.WOMERSLEY, (0.0, 0.0, -5.0), (0.0, 0.0, 1.0), radius=radius, period=period,
+                   pressure_gradient_amplitude=pga, womersley_number=alpha)
+
+
+def test_womersley_reference_expectations():
+    """InOutLetTests.h:226-274: zero at the wall; at alpha = 1e-4 the
+    centreline speed after a quarter period is the Poiseuille G R^2 / 4 eta
+    and reverses after three quarters (eta = nu = 1)."""
+    io = womersley(2.0, 5.0)
+    assert np.abs(O.iolet_velocity(io, (10.0, 0.0, -5.0), 0)).max() < 1e-9
+    alpha, R = 1e-4, 10.0
+    period = 2 * math.pi * R ** 2 / (alpha * alpha * 1.0)
+    io = womersley(alpha, period)
+    pois = MMHG * 1e-6 * R ** 2 / 4
+    v = O.iolet_velocity(io, (0.0, 0.0, -5.0), int(0.25 * period))
+    assert abs(v[0]) < 1e-9 and abs(v[1]) < 1e-9 and abs(v[2] - pois) < 1e-9
+    v = O.iolet_velocity(io, (0.0, 0.0, -5.0), int(0.75 * period))
+    assert abs(v[2] + pois) < 1e-9
+    # and the whole quasi-steady profile is parabolic
+    for r in (2.0, 5.0, 8.0):
+        v = O.iolet_velocity(io, (r, 0.0, -5.0), int(0.25 * period))
+        assert abs(v[2] - MMHG * 1e-6 * (R * R - r * r) / 4) < 1e-9
+
+
+CASE_XML = """<?xml version="1.0"?>
+<hemelbsettings version="3">
+  <simulation>
+    <step_length value="0.0001" units="s"/>
+    <steps value="40000" units="lattice"/>
+    <stresstype value="1"/>
+    <voxel_size value="0.01" units="m"/>
+    <origin value="(0.001,0.002,0.003)" units="m"/>
+  </simulation>
+  <geometry><datafile path="four_cube.gmy"/></geometry>
+  <initialconditions>
+    <time value="100" units="lattice"/>
+    <checkpoint file="cp.xtr"/>
+  </initialconditions>
+  <inlets>
+    <inlet>
+      <condition type="pressure" subtype="file"><path value="iolet_pressure.txt"/></condition>
+      <normal value="(0.0,0.0,1.0)" units="dimensionless"/>
+      <position value="(2.5,2.5,0.5)" units="lattice"/>
+    </inlet>
+    <inlet>
+      <condition type="velocity" subtype="womersley">
+        <pressure_gradient_amplitude value="1e3" units="mmHg/m"/>
+        <period value="0.0005" units="s"/>
+        <womersley_number value="2" units="dimensionless"/>
+        <radius value="0.1" units="m"/>
+      </condition>
+      <normal value="(0.0,0.0,1.0)" units="dimensionless"/>
+      <position value="(0,0,-5)" units="lattice"/>
+    </inlet>
+  </inlets>
+  <outlets>
+    <outlet>
+      <condition type="pressure" subtype="WK3">
+        <Rc value="1e7" units="kg/m^4*s"/><Rp value="1e8" units="kg/m^4*s"/>
+        <Cp value="1e-9" units="m^4*s^2/kg"/><radius value="0.02" units="m"/><area value="1e-4" units="m^2"/>
+      </condition>
+      <normal value="(0.0,0.0,-1.0)" units="dimensionless"/>
+      <position value="(2.5,2.5,4.5)" units="lattice"/>
+    </outlet>
+    <outlet>
+      <condition type="pressure" subtype="fileWK">
+        <path value="unused.txt"/>
+        <R value="2e7" units="kg/m^4*s"/><C value="3e-9" units="m^4*s^2/kg"/><area value="2e-4" units="m^2"/>
+      </condition>
+      <normal value="(0.0,0.0,-1.0)" units="dimensionless"/>
+      <position value="(2.5,2.5,4.5)" units="lattice"/>
+    </outlet>
+  </outlets>
+  <properties>
+    <propertyoutput file="whole.xtr" period="100">
+      <geometry type="whole"/>
+      <field type="velocity"/>
+      <field type="pressure" name="p"/>
+    </propertyoutput>
+    <propertyoutput file="line.xtr" period="50" start="10" stop="1000">
+      <geometry type="line"><point value="(0,0,0)" units="m"/><point value="(0.01,0,0)" units="m"/></geometry>
+      <field type="stresstensor"/>
+    </propertyoutput>
+  </properties>
+</hemelbsettings>
+"""
+
+
+def test_case_reader_converts_every_iolet_and_output(tmp_path):
+    """hemepure_amd.config.load_case on a case with a file pressure inlet, a
+    Womersley inlet (the reference test's parameters: radius 10, gradient
+    mmHg_TO_PASCAL * 1e-6, period 5 in lattice units, InOutLetTests.h:217-221),
+    WK3 and fileWK outlets, a checkpoint initial condition and two outputs."""
+    from hemepure_amd.config import load_case
+
+    (tmp_path / "case.xml").write_text(CASE_XML)
+    (tmp_path / "iolet_pressure.txt").write_text((GOLDEN / "iolet_pressure.txt").read_text())
+    c = load_case(tmp_path / "case.xml")
+    fp, wo = c.inlets
+    assert fp.kind == H.FILEPRES and fp.table_len == 40001
+    table, dmin = H.iolet_file_table(GOLDEN / "iolet_pressure.txt", False, 40000, 1e-4, 0.01)
+    assert np.array_equal(np.ctypeslib.as_array(fp.table, (40001,)), table) and fp.density_min == dmin
+    assert wo.kind == H.WOMERSLEY and wo.radius == pytest.approx(10.0, rel=1e-15)
+    assert wo.pressure_gradient_amplitude == pytest.approx(MMHG * 1e-6, rel=1e-14)
+    assert wo.period == pytest.approx(5.0, rel=1e-14) and wo.womersley_number == 2.0
+    wk3, fwk = c.outlets
+    dt, dx = 1e-4, 0.01
+    assert wk3.kind == H.WK3 and wk3.r_c == pytest.approx(1e7 * dt * dx / 1000) and \
+        wk3.r_p == pytest.approx(1e8 * dt * dx / 1000)
+    assert wk3.capacitance == pytest.approx(1e-9 * 1000 / (dt * dt * dx)) and wk3.area == pytest.approx(1.0)
+    assert wk3.radius == pytest.approx(2.0)
+    assert fwk.kind == H.WK2 and fwk.resistance == pytest.approx(2e7 * dt * dx / 1000) and \
+        fwk.area == pytest.approx(2.0)
+    assert c.t0 == 100 and c.checkpoint == tmp_path / "cp.xtr"
+    assert c.units.origin == (0.001, 0.002, 0.003)
+    whole, line = c.outputs
+    assert whole.selector == H.SELECT_WHOLE and whole.fields == [(H.FIELD_VELOCITY, "velocity"), (H.FIELD_PRESSURE, "p")]
+    assert line.selector == H.SELECT_LINE and line.point2 == (0.01, 0.0, 0.0) and line.start == 10
+    assert line.due(100) and not line.due(1050) and not line.due(75)
