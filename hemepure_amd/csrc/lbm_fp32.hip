@@ -239,8 +239,9 @@ __device__ __forceinline__ int member_of_tile(const int32_t* wg_member, uint32_t
 }
 
 // B threads (B / 64 slabs) per workgroup tile; REMAP: XCD-contiguous tile order.
-template <int B, bool REMAP>
-__global__ __launch_bounds__(B) void step_fp32(const Member32* __restrict__ members,
+template <int B, bool REMAP, int W = 0>
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1))) void step_fp32(
+    const Member32* __restrict__ members,
                                                const int32_t* __restrict__ wg_member, int step) {
   const uint32_t tile = REMAP ? xcd_tile() : blockIdx.x;
   const int mi = member_of_tile(wg_member, tile);
@@ -486,11 +487,11 @@ int fp32_variant_block(int variant) {
 }
 int fp32_variant_slabs(int variant) { return fp32_variant_block(variant) / kSlab; }
 bool fp32_variant_remap(int variant) { return ((variant % kFp32Shapes) & 1) != 0; }
+int fp32_variant_level(int variant) { return variant / kFp32Shapes; }
 unsigned fp32_variant_lds(int variant) {
   // 160 KB of LDS per CU, four SIMDs: w waves per SIMD <=> 160 KB / (4 w) per wave.
-  static const unsigned per_wave[4] = {0u, 10240u, 13653u, 20480u};
-  const int level = variant / kFp32Shapes;
-  return per_wave[level] * (unsigned)(fp32_variant_block(variant) / 64);
+  static const unsigned per_wave[6] = {0u, 10240u, 13653u, 20480u, 0u, 0u};
+  return per_wave[fp32_variant_level(variant)] * (unsigned)(fp32_variant_block(variant) / 64);
 }
 
 // `grid` workgroup tiles of the variant's size, padded to a multiple of the
@@ -499,15 +500,23 @@ hipError_t launch_step_fp32(const Member32* members, const int32_t* wg_member, u
                             int variant, hipStream_t s) {
   if (grid == 0) return hipSuccess;
   const unsigned lds = fp32_variant_lds(variant);
-#define HLB_V(B, R) hipLaunchKernelGGL((step_fp32<B, R>), dim3(grid), dim3(B), lds, s, members, wg_member, step)
-  switch (variant % kFp32Shapes) {
-    case 0: HLB_V(128, false); break;
-    case 1: HLB_V(128, true); break;
-    case 2: HLB_V(256, false); break;
-    case 3: HLB_V(256, true); break;
-    case 4: HLB_V(512, false); break;
-    default: HLB_V(512, true); break;
+#define HLB_V(B, R, W) hipLaunchKernelGGL((step_fp32<B, R, W>), dim3(grid), dim3(B), lds, s, members, wg_member, step)
+#define HLB_SHAPES(W)                \
+  switch (variant % kFp32Shapes) {   \
+    case 0: HLB_V(128, false, W); break; \
+    case 1: HLB_V(128, true, W); break;  \
+    case 2: HLB_V(256, false, W); break; \
+    case 3: HLB_V(256, true, W); break;  \
+    case 4: HLB_V(512, false, W); break; \
+    default: HLB_V(512, true, W); break; \
   }
+  // the FP32 kernel fits 5 waves per SIMD as compiled; levels 4, 5 ask for 6, 8
+  switch (fp32_variant_level(variant)) {
+    case 4: HLB_SHAPES(6); break;
+    case 5: HLB_SHAPES(8); break;
+    default: HLB_SHAPES(0); break;
+  }
+#undef HLB_SHAPES
 #undef HLB_V
   return hipGetLastError();
 }

@@ -228,8 +228,8 @@ __device__ __forceinline__ uint32_t slab_record() {
 }
 
 // B threads (B / 64 slabs) per workgroup; REMAP: XCD-contiguous tile order.
-template <int KERNEL, bool FIRST, bool PROPS, int B = kBlock, bool REMAP = true>
-__global__ __launch_bounds__(B) void step_fp64(StepArgs a) {
+template <int KERNEL, bool FIRST, bool PROPS, int B = kBlock, bool REMAP = true, int W = 0>
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1))) void step_fp64(StepArgs a) {
   Lane L;
   const uint32_t tile = REMAP ? xcd_tile() : blockIdx.x;
   if (!slab_lane(a.bt, tile * (B / kSlab) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), L)) return;
@@ -663,15 +663,23 @@ void launch_variant_fp64(const StepArgs& a, int variant, hipStream_t s) {
   const int64_t tiles = (a.bt.nslab + per - 1) / per;
   const dim3 grid((unsigned)((tiles + kXcds - 1) / kXcds * kXcds));
   const unsigned lds = fp32_variant_lds(variant);
-#define HLB_V(B, R) hipLaunchKernelGGL((step_fp64<KN, false, false, B, R>), grid, dim3(B), lds, s, a)
-  switch (variant % kFp32Shapes) {
-    case 0: HLB_V(128, false); break;
-    case 1: HLB_V(128, true); break;
-    case 2: HLB_V(256, false); break;
-    case 3: HLB_V(256, true); break;
-    case 4: HLB_V(512, false); break;
-    default: HLB_V(512, true); break;
+#define HLB_V(B, R, W) hipLaunchKernelGGL((step_fp64<KN, false, false, B, R, W>), grid, dim3(B), lds, s, a)
+#define HLB_SHAPES(W)                    \
+  switch (variant % kFp32Shapes) {       \
+    case 0: HLB_V(128, false, W); break; \
+    case 1: HLB_V(128, true, W); break;  \
+    case 2: HLB_V(256, false, W); break; \
+    case 3: HLB_V(256, true, W); break;  \
+    case 4: HLB_V(512, false, W); break; \
+    default: HLB_V(512, true, W); break; \
   }
+  // the FP64 kernel fits 4 waves per SIMD as compiled; levels 4, 5 ask for 5, 6
+  switch (fp32_variant_level(variant)) {
+    case 4: HLB_SHAPES(5); break;
+    case 5: HLB_SHAPES(6); break;
+    default: HLB_SHAPES(0); break;
+  }
+#undef HLB_SHAPES
 #undef HLB_V
 }
 

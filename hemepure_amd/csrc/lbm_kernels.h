@@ -56,14 +56,17 @@ constexpr int kMaxIolets = 256;  // iolets per lattice
 // without and with the XCD-contiguous tile order.  Level 0..3 caps the
 // waves resident per SIMD (none, 4, 3, 2) by giving each workgroup dynamic
 // LDS it does not use: fewer waves in flight keep more of each wave's
-// neighbour lines in the XCD's L2.
+// neighbour lines in the XCD's L2.  Levels 4 and 5 raise the floor instead:
+// the kernel is compiled for at least fp32_variant_waves() waves per SIMD
+// (fewer registers, some spilled), for more loads in flight.
 constexpr int kFp32Shapes = 6;
-constexpr int kFp32Variants = 24;
+constexpr int kFp32Variants = 36;
 constexpr int kFp32DefaultVariant = 1;  // 128 slots (two slabs), XCD order, no cap
 int fp32_variant_block(int variant);
 int fp32_variant_slabs(int variant);  // slabs per workgroup tile
 bool fp32_variant_remap(int variant);
 unsigned fp32_variant_lds(int variant);  // dynamic LDS bytes per workgroup
+int fp32_variant_level(int variant);      // 0..5
 
 // Guo-Zheng-Shi wall links (GuoZhengShiDelegate.h:123-294).  Sites whose
 // pre-collision f the link kernel needs (GZS wall sites and the next fluid

@@ -175,9 +175,11 @@ int hlb_upload_f(hlb_handle h, const double* f, int64_t t0);
  * during the last step (BaseStreamer::UpdateMinsAndMaxes,
  * src/lb/streamers/BaseStreamer.h:88-120).  Asynchronous. */
 int hlb_step(hlb_handle h, int64_t nsteps, uint32_t flags);
-/* Launch variant of the steady-state step, 0..23 = shape + 6 * level: shape
+/* Launch variant of the steady-state step, 0..35 = shape + 6 * level: shape
  * 0..5 = workgroups of 128/256/512 slots without/with the XCD-contiguous
- * order, level 0..3 = at most (unlimited, 4, 3, 2) waves per SIMD; default 1.
+ * order, level 0..3 = at most (unlimited, 4, 3, 2) waves per SIMD, level
+ * 4..5 = compiled for at least (5, 6) waves per SIMD in FP64 and (6, 8) in
+ * the FP32-shifted form; default 1.
  * No counterpart in the reference.  hlb_autotune times every variant over
  * trial_steps real steps of the lattice (they advance it) and keeps the
  * fastest; results do not depend on the variant. */
@@ -321,7 +323,7 @@ int hlb_batch_step(hlb_batch_handle b, int64_t nsteps, uint32_t flags);
 int hlb_batch_step_timed(hlb_batch_handle b, int64_t nsteps, uint32_t flags, double* ms);
 /* Step sequences are captured into HIP graphs and replayed (default on). */
 int hlb_batch_set_graphs(hlb_batch_handle b, int enable);
-/* Launch variant, as for hlb_set_variant (0..23). */
+/* Launch variant, as for hlb_set_variant (0..35). */
 int hlb_batch_set_variant(hlb_batch_handle b, int variant);
 int hlb_batch_variant(hlb_batch_handle b);
 /* Times each launch variant over trial_steps real steps of every member and

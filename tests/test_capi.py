@@ -88,7 +88,7 @@ def test_batch_rejects_fp64_member_without_device():
     with pytest.raises(H.HlbError, match="FP32"):
         b.add(lat)
     with pytest.raises(H.HlbError, match="no such variant"):
-        b.set_variant(24)
+        b.set_variant(36)
 
 
 def test_server_validates_requests_without_device():

@@ -110,7 +110,7 @@ def test_every_launch_shape_and_autotune():
         total += 7
     v, ms = b.autotune(5)
     total += 24 * 2 * 5
-    assert 0 <= v < 24 and ms > 0 and b.variant() == v
+    assert 0 <= v < 36 and ms > 0 and b.variant() == v
     b.step(9, properties=True)
     total += 9
     for i, (c, l) in enumerate(zip(cases, lats)):
