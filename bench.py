@@ -118,7 +118,7 @@ def measure_traffic(workload: str, variant: int = 1) -> dict:
     tmp = Path(os.environ.get("TMPDIR", "/tmp")) / f"hlb_pmc_{os.getpid()}"
     me = [sys.executable, "-u", str(Path(__file__).resolve())]
     runs = {"bench": me + ["--steps", "10", "--warmup", "2", "--no-cpu-baseline", "--no-ensemble", "--no-serving",
-                           "--workload", workload, "--variant", str(variant)],
+                           "--no-vessel-tree", "--workload", workload, "--variant", str(variant)],
             "cal": me + ["--calibrate"]}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         for tag, cmd in runs.items():
@@ -268,6 +268,34 @@ def bench_ensemble(members: int, steps: int, solve_steps: int, workers: int):
     }
 
 
+def bench_vessel_tree(steps: int, warmup: int, threads: int):
+    """The sparse vessel tree (63 tubes, root r 40 tapering 0.79, branch
+    angles 20-45 deg, box 1024 x 1024 x 768, LES around tau 0.505, WK3
+    outlets, cosine pressure inlet), one instance
+    timed over `steps` steps in the FP32-shifted form and the FP64 form.
+    Voxelising the geometry is not timed."""
+    import hemepure_amd as H
+    from fixtures import vessel_tree_case
+
+    t0 = time.perf_counter()
+    case = vessel_tree_case(threads=threads)
+    n = case.geom.n_sites
+    log(f"vessel tree: {n} sites, {len(case.outlets)} WK3 outlets, voxelised in {time.perf_counter() - t0:.1f} s")
+    out = {"workload": "binary vessel tree of 63 tubes, root r40 x0.79 per generation, length 6r, angles "
+                       "U(20,45) deg, seed 2026, box 1024x1024x768; LBGK+LES (C_s 0.1) tau 0.505; SBB walls; "
+                       "cosine pressure inlet rho 1.0005 +- 5e-4 period 1000; 32 WK3 outlets R1=1e-3 R2=100 C=1000 "
+                       "(R2=1e-3, C=10 empty the outlets under the absolute-pressure WK3 update)",
+           "sites": n, "box_fill": round(n / (1024 * 1024 * 768), 5), "steps_timed": steps}
+    for key, prec, kind in (("fp32_shifted", H.FP32_SHIFTED, "f32"), ("fp64", H.FP64, "f64")):
+        n_, ms, stable, variant = time_form(case, prec, steps, warmup)
+        ms_step = ms / steps
+        out[key] = {"mflups": round(n_ * steps / (ms / 1e3) / 1e6, 2), "ms_per_step": ms_step,
+                    "hbm_frac": round(n_ * BYTES_PER_SITE[kind] / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "launch_variant": variant, "stable": bool(stable)}
+        log(f"vessel tree {key}: {out[key]['mflups']} MFLUPS" + ("" if stable else " (unstable)"))
+    return out
+
+
 def bench_serving(count: int, rate: float, max_sites: float, workers: int, quantum: int = 500):
     """The serving stream: `count` requests (pipes, stenoses, bifurcations,
     20k..max_sites sites, LBGK or TRT) arriving as a Poisson process of
@@ -358,6 +386,8 @@ def main() -> int:
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream measurement")
+    ap.add_argument("--no-vessel-tree", action="store_true", help="skip the sparse vessel-tree measurement")
+    ap.add_argument("--vessel-tree-steps", type=int, default=1000)
     ap.add_argument("--serving-requests", type=int, default=24)
     ap.add_argument("--serving-rate", type=float, default=8.0, help="Poisson arrivals per second")
     ap.add_argument("--serving-max-sites", type=float, default=5e6)
@@ -380,7 +410,7 @@ def main() -> int:
 
     import hemepure_amd as H
 
-    total = args.steps + args.warmup + 8 + 6 * 2 * 10 + 64  # + the launch-shape trials
+    total = args.steps + args.warmup + 8 + 36 * 2 * 10 + 64  # + the launch-variant trials
     case, desc = make_case(args.workload, total)
     log(f"workload {args.workload}: {case.geom.n_sites} sites")
     lat = case.product()
@@ -447,6 +477,8 @@ def main() -> int:
     }
     if not args.no_ensemble:
         out["ensemble"] = bench_ensemble(args.ensemble_members, args.ensemble_steps, 5000, args.workers)
+    if not args.no_vessel_tree:
+        out["vessel_tree"] = bench_vessel_tree(args.vessel_tree_steps, args.warmup, args.workers)
     if not args.no_serving:
         out["serving"] = bench_serving(args.serving_requests, args.serving_rate, args.serving_max_sites,
                                        args.workers)

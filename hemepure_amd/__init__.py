@@ -210,6 +210,8 @@ _SIGS = {
                                         C.c_double, C.c_uint64, C.POINTER(C.c_void_p)]),
     "hlb_geom_stenosed_pipe": (C.c_int, [C.c_double, C.c_int32, C.c_double, C.c_double,
                                          C.POINTER(C.c_void_p)]),
+    "hlb_geom_vessel_tree": (C.c_int, [C.c_uint64, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_void_p)]),
     "hlb_geom_view": (C.c_int, [C.c_void_p, C.POINTER(GeometryView)]),
     "hlb_geom_read_gmy": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]),
     "hlb_geom_iolets": (C.c_int, [C.c_void_p, C.POINTER(IoletGeom), C.c_int]),
@@ -329,6 +331,15 @@ class Geometry:
     def stenosed_pipe(cls, radius: float, length: int, area_reduction: float, width: float) -> "Geometry":
         g = C.c_void_p()
         _check(lib().hlb_geom_stenosed_pipe(radius, length, area_reduction, width, C.byref(g)))
+        return cls._from_native(g)
+
+    @classmethod
+    def vessel_tree(cls, seed=2026, generations=6, root_radius=40.0, taper=0.79, length_factor=6.0,
+                    angle_min=20.0, angle_max=45.0, box=(1024, 1024, 768), threads=0) -> "Geometry":
+        g = C.c_void_p()
+        b = (C.c_int32 * 3)(*box)
+        _check(lib().hlb_geom_vessel_tree(int(seed), int(generations), root_radius, taper, length_factor, angle_min,
+                                          angle_max, b, int(threads), C.byref(g)))
         return cls._from_native(g)
 
     def inlets(self):

@@ -452,6 +452,17 @@ int hlb_geom_ybifurcation(double rp, double lp, double rd, double ld, double ang
  * mid-length, width parameter w (lattice units). */
 int hlb_geom_stenosed_pipe(double radius, int32_t length, double area_reduction, double width,
                            hlb_geom** out);
+/* Binary tree of tapering tubes in a box (multiples of 8): the root (radius
+ * root_radius) enters at the z = -0.5 face along +z, every segment end splits
+ * into two children of radius taper * r turned by U(angle_min, angle_max)
+ * degrees to opposite sides of a random plane, segments are length_factor *
+ * r long (shortened to stay two radii inside the box).  generations levels
+ * give 2^generations - 1 segments; inlet 0 at the root, outlets 0.. at the
+ * leaves.  Voxelised on nthreads host threads (0: all cores); the result
+ * does not depend on the thread count. */
+int hlb_geom_vessel_tree(uint64_t seed, int32_t generations, double root_radius, double taper,
+                         double length_factor, double angle_min_deg, double angle_max_deg,
+                         const int32_t box[3], int32_t nthreads, hlb_geom** out);
 /* Reads a HemeLB geometry file, version 4 (gmy_plus = 1: the .gmy+ variant
  * with per-block weights); replaces GeometryReader::LoadAndDecompose's
  * reading (src/geometry/GeometryReader.cc:382-446, 851-935) on one process.

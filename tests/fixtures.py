@@ -289,3 +289,27 @@ def bifurcation_case(rp=24.0, lp=200.0, rd=19.0, ld=160.0, steps=8000, peak=0.05
                 tau=tau, trt_lambda=3.0 / 16.0,
                 inlets=[IoletSpec(H.FILEVEL, i0["position"], i0["normal"], radius=i0["radius"], table=table)],
                 outlets=outs)
+
+
+def vessel_tree_case(seed=2026, generations=6, root_radius=40.0, box=(1024, 1024, 768), length_factor=6.0,
+                     tau=0.505, rho_mean=1.0005, rho_amp=5e-4, period=1000.0, r_c=1e-3, r_p=100.0,
+                     capacitance=1000.0, threads=0) -> Case:
+    """The sparse vessel tree: 2^generations - 1 tubes from the root radius
+    tapering by 0.79 per generation, branch angles U(20, 45) deg, LBGK with
+    the LES (Smagorinsky C_s 0.1) relaxation time around tau, SBB walls, a
+    cosine pressure inlet and three-element Windkessel outlets (R1 = r_c,
+    R2 = r_p, C; area pi r^2 of the leaf).  The WK3 update works on the
+    absolute lattice pressure (InOutLetWK3.cc:80-90), so R2 C sets how fast
+    the outlet pressure relaxes toward (R1 + R2) Q: with R2 = 1e-3, C = 10 it
+    reaches ~0 within a few steps and the lattice empties; R2 C = 1e5 steps
+    keeps the outlets near rho = 1 over a 1000-step run."""
+    g = H.Geometry.vessel_tree(seed=seed, generations=generations, root_radius=root_radius, taper=0.79,
+                               length_factor=length_factor, angle_min=20.0, angle_max=45.0, box=box,
+                               threads=threads)
+    i0 = g.inlets()[0]
+    outs = [IoletSpec(H.WK3, o["position"], o["normal"], radius=o["radius"], area=math.pi * o["radius"] ** 2,
+                      r_c=r_c, r_p=r_p, capacitance=capacitance) for o in g.outlets()]
+    return Case(g, kernel=H.KERNEL_LES, wall_bc=H.WALL_SBB, tau=tau,
+                inlets=[IoletSpec(H.COSINE, i0["position"], i0["normal"], density_mean=rho_mean, density_amp=rho_amp,
+                                  period=period)],
+                outlets=outs)

@@ -104,12 +104,12 @@ def test_every_launch_shape_and_autotune():
     lats = [make(c) for c in cases]
     b = H.Batch(lats)
     total = 0
-    for v in (0, 1, 2, 3, 4, 5, 7, 14, 21, 23):
+    for v in (0, 1, 2, 3, 4, 5, 7, 14, 21, 23, 25, 30, 35):
         b.set_variant(v)
         b.step(7)
         total += 7
     v, ms = b.autotune(5)
-    total += 24 * 2 * 5
+    total += 36 * 2 * 5  # every launch variant, two timed trials each
     assert 0 <= v < 36 and ms > 0 and b.variant() == v
     b.step(9, properties=True)
     total += 9

@@ -71,3 +71,22 @@ def test_womersley_fp32_close_to_oracle():
     ref.run(300, properties=True, threads=0)
     e = field_errors(lat, ref, "womersley fp32")
     assert e["u"] <= 1e-4 and e["rho"] <= 1e-4
+
+
+def test_small_vessel_tree_les_windkessel():
+    """The vessel-tree configuration in miniature (7 tubes, root r 8, box
+    96^3): LES around tau 0.505, SBB, cosine inlet, four WK3 outlets; FP64
+    bit-exact against the oracle, FP32-shifted within 1e-4."""
+    from fixtures import vessel_tree_case
+    from test_gpu_fp32 import field_errors
+
+    case = vessel_tree_case(generations=3, root_radius=8.0, box=(96, 96, 96), threads=4, period=200.0)
+    lat, ref = run_both(case, 400)
+    compare_all(lat, ref, "vessel tree")
+    for k in range(len(case.outlets)):
+        assert np.array_equal(lat.iolet_state(True, k), ref.iolet_state(True, k))
+    lat32 = case.product(precision=H.FP32_SHIFTED)
+    lat32.init_equilibrium(1.0)
+    lat32.step(400, properties=True)
+    e = field_errors(lat32, ref, "vessel tree fp32")
+    assert e["u"] <= 1e-4 and e["rho"] <= 1e-4

@@ -324,18 +324,19 @@ def test_launch_shapes_do_not_change_results(precision):
     """Every launch variant (shape and occupancy cap) and the autotuner's
     choice give the same bits."""
     case = bifurcation_case(rp=8.0, lp=40.0, rd=6.5, ld=32.0, steps=120, peak=0.03, period=200.0)
+    steps = 36 * 2 + 2  # the autotuner's trials (36 variants x 2 of one step) and two more
     ref = case.product(precision=precision)
     ref.init_equilibrium(1.0)
-    ref.step(50, properties=True)
+    ref.step(steps, properties=True)
     want = ref.download_f()
     for v in (0, 1, 2, 3, 4, 5, 7, 14, 21, 23, 25, 30, 35):
         lat = case.product(precision=precision)
         lat.set_variant(v)
         lat.init_equilibrium(1.0)
-        lat.step(50, properties=True)
+        lat.step(steps, properties=True)
         assert np.array_equal(lat.download_f(), want), f"variant {v}"
     lat = case.product(precision=precision)
     lat.init_equilibrium(1.0)
-    v, ms = lat.autotune(1)  # 36 variants x 2 timed trials of one step
+    v, ms = lat.autotune(1)
     lat.step(2, properties=True)
     assert 0 <= v < 36 and ms > 0 and np.array_equal(lat.download_f(), want)
