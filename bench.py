@@ -283,8 +283,8 @@ def bench_vessel_tree(steps: int, warmup: int, threads: int):
     log(f"vessel tree: {n} sites, {len(case.outlets)} WK3 outlets, voxelised in {time.perf_counter() - t0:.1f} s")
     out = {"workload": "binary vessel tree of 63 tubes, root r40 x0.79 per generation, length 6r, angles "
                        "U(20,45) deg, seed 2026, box 1024x1024x768; LBGK+LES (C_s 0.1) tau 0.505; SBB walls; "
-                       "cosine pressure inlet rho 1.0005 +- 5e-4 period 1000; 32 WK3 outlets R1=1e-3 R2=100 C=1000 "
-                       "(R2=1e-3, C=10 empty the outlets under the absolute-pressure WK3 update)",
+                       "cosine pressure inlet rho 1.0001 +- 1e-4 period 1000; 32 WK3 outlets R1=R2=1e-3 C=1e9 "
+                       "(C=10 empties the outlets within steps under the absolute-pressure WK3 update)",
            "sites": n, "box_fill": round(n / (1024 * 1024 * 768), 5), "steps_timed": steps}
     for key, prec, kind in (("fp32_shifted", H.FP32_SHIFTED, "f32"), ("fp64", H.FP64, "f64")):
         n_, ms, stable, variant = time_form(case, prec, steps, warmup)

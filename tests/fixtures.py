@@ -292,17 +292,18 @@ def bifurcation_case(rp=24.0, lp=200.0, rd=19.0, ld=160.0, steps=8000, peak=0.05
 
 
 def vessel_tree_case(seed=2026, generations=6, root_radius=40.0, box=(1024, 1024, 768), length_factor=6.0,
-                     tau=0.505, rho_mean=1.0005, rho_amp=5e-4, period=1000.0, r_c=1e-3, r_p=100.0,
-                     capacitance=1000.0, threads=0) -> Case:
+                     tau=0.505, rho_mean=1.0001, rho_amp=1e-4, period=1000.0, r_c=1e-3, r_p=1e-3,
+                     capacitance=1e9, threads=0) -> Case:
     """The sparse vessel tree: 2^generations - 1 tubes from the root radius
     tapering by 0.79 per generation, branch angles U(20, 45) deg, LBGK with
     the LES (Smagorinsky C_s 0.1) relaxation time around tau, SBB walls, a
     cosine pressure inlet and three-element Windkessel outlets (R1 = r_c,
     R2 = r_p, C; area pi r^2 of the leaf).  The WK3 update works on the
-    absolute lattice pressure (InOutLetWK3.cc:80-90), so R2 C sets how fast
-    the outlet pressure relaxes toward (R1 + R2) Q: with R2 = 1e-3, C = 10 it
-    reaches ~0 within a few steps and the lattice empties; R2 C = 1e5 steps
-    keeps the outlets near rho = 1 over a 1000-step run."""
+    absolute lattice pressure (InOutLetWK3.cc:80-90): the outlet pressure
+    relaxes toward (R1 + R2) Q with time constant R2 C.  With R1 = R2 = 1e-3
+    and C = 10 that is ~0 within a few steps (the outlets empty and the run
+    blows up); C = 1e9 gives R2 C = 1e6 steps, so over a few thousand steps
+    the outlets stay near rho = 1 while still running the WK3 update."""
     g = H.Geometry.vessel_tree(seed=seed, generations=generations, root_radius=root_radius, taper=0.79,
                                length_factor=length_factor, angle_min=20.0, angle_max=45.0, box=box,
                                threads=threads)
