@@ -496,24 +496,28 @@ int upload_lattice(hlb_lattice* h) {
       }
     }
   }
-  // Brick tables.
+  // Slab records: masks, boundary base and the reachable bricks.
   {
-    const size_t ns = std::max<size_t>(1, h->h_slab.size());
-    HLB_CHECK(h->nbr.alloc(h->h_nbr.size()));
-    HLB_CHECK(h->slab.alloc(ns));
-    HLB_CHECK(h->fmask.alloc(ns));
-    HLB_CHECK(h->bmask.alloc(ns));
-    HLB_CHECK(h->bbase.alloc(ns));
-    HLB_CHECK(hipMemcpy(h->nbr.p, h->h_nbr.data(), sizeof(int32_t) * h->h_nbr.size(), hipMemcpyHostToDevice));
-    if (!h->h_slab.empty()) {
-      HLB_CHECK(hipMemcpy(h->slab.p, h->h_slab.data(), sizeof(int32_t) * h->h_slab.size(), hipMemcpyHostToDevice));
-      HLB_CHECK(hipMemcpy(h->fmask.p, h->h_fmask.data(), sizeof(uint64_t) * h->h_fmask.size(),
-                          hipMemcpyHostToDevice));
-      HLB_CHECK(hipMemcpy(h->bmask.p, h->h_bmask.data(), sizeof(uint64_t) * h->h_bmask.size(),
-                          hipMemcpyHostToDevice));
-      HLB_CHECK(hipMemcpy(h->bbase.p, h->h_bbase.data(), sizeof(int32_t) * h->h_bbase.size(),
-                          hipMemcpyHostToDevice));
+    const size_t ns = h->h_slab.size();
+    const int32_t empty = (int32_t)h->nbrick;
+    std::vector<SlabRec> recs(std::max<size_t>(1, ns));
+    for (size_t r = 0; r < ns; ++r) {
+      SlabRec& R = recs[r];
+      std::memset(&R, 0, sizeof R);
+      R.sid = h->h_slab[r];
+      R.bbase = h->h_bbase[r];
+      R.fmask = h->h_fmask[r];
+      R.bmask = h->h_bmask[r];
+      const int64_t brick = R.sid >> 3;
+      const int x = R.sid & 7;
+      const int dxo = x == 0 ? -1 : (x == 7 ? 1 : 0);
+      for (int k = 0; k < 9; ++k) {
+        R.nb[k] = h->h_nbr[(size_t)brick * 27 + 9 + k];
+        R.nb[9 + k] = dxo ? h->h_nbr[(size_t)brick * 27 + (dxo + 1) * 9 + k] : empty;
+      }
     }
+    HLB_CHECK(h->rec.alloc(recs.size()));
+    HLB_CHECK(hipMemcpy(h->rec.p, recs.data(), sizeof(SlabRec) * recs.size(), hipMemcpyHostToDevice));
   }
   // Boundary arrays in device order.
   {
@@ -1303,7 +1307,7 @@ int64_t device_bytes(const hlb_lattice* h) {
   const int64_t S = h->S, SB = h->SB, SI = h->SI;
   const int64_t fsz = h->fp32() ? 4 : 8;
   int64_t b = 3 * Q * fsz * S + Q * fsz * SI;        // A[2], F, G
-  b += 18 * 4 * S + 4 * S;                            // pull, bidx
+  b += (int64_t)sizeof(SlabRec) * (S / kSlab);       // slab records (at most one per 64 slots)
   b += (4 * 4 + 18 * 4 + 3 * 4) * SB;                 // boundary masks, ids, q, normals
   b += 36 * 8 * SI + 8 * SI;                          // Ladd factors, Windkessel terms
   b += 5 * 8 * S;                                     // rho, u, wss

@@ -64,18 +64,23 @@ struct Lane {
   int x, y, z;       // x is wave-uniform
   bool boundary;
   uint32_t b;        // compact boundary index (valid when boundary)
-  const int32_t* row;  // the brick's 27 neighbour bricks (wave-uniform)
+  int32_t nb[18];    // the slab's reachable bricks (wave-uniform), as in SlabRec
 };
 
 // Decodes slab record r for this lane; false for a hole or a record past the
-// end (the lane has nothing to do).  r must be wave-uniform.
+// end (the lane has nothing to do).  r must be wave-uniform; the record is
+// read with scalar loads that do not wait on one another.
 __device__ __forceinline__ bool slab_lane(const BrickTables& bt, uint32_t r, Lane& L) {
   if (r >= (uint64_t)bt.nslab) return false;
   const uint32_t lane = threadIdx.x & 63u;
-  const int32_t sid = __builtin_amdgcn_readfirstlane(ldp(bt.slab, r));
-  const uint64_t fm = uniform64(ldp(bt.fmask, r));
+  const HLB_GLOBAL SlabRec* g = (const HLB_GLOBAL SlabRec*)(bt.rec + r);
+  const uint64_t fm = uniform64(g->fmask);
+  const uint64_t bm = uniform64(g->bmask);
+  const int32_t sid = __builtin_amdgcn_readfirstlane(g->sid);
+  const int32_t bbase = __builtin_amdgcn_readfirstlane(g->bbase);
+#pragma unroll
+  for (int k = 0; k < 18; ++k) L.nb[k] = __builtin_amdgcn_readfirstlane(g->nb[k]);
   if (!((fm >> lane) & 1u)) return false;
-  const uint64_t bm = uniform64(ldp(bt.bmask, r));
   const uint32_t brick = (uint32_t)sid >> 3;
   L.r = r;
   L.x = sid & 7;
@@ -86,9 +91,8 @@ __device__ __forceinline__ bool slab_lane(const BrickTables& bt, uint32_t r, Lan
   L.b = 0;
   if (L.boundary) {
     const uint64_t below = bm & ((1ull << lane) - 1ull);
-    L.b = (uint32_t)__builtin_amdgcn_readfirstlane(ldp(bt.bbase, r)) + (uint32_t)__builtin_popcountll(below);
+    L.b = (uint32_t)bbase + (uint32_t)__builtin_popcountll(below);
   }
-  L.row = bt.nbr + (int64_t)brick * 27;
   return true;
 }
 
@@ -99,28 +103,34 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 }
 
 // Slot of the upstream site x - c_J of a lane (pull direction J).  The brick
-// holding it is one of at most four candidates read from the uniform
-// neighbour row; the lane picks by whether its y / z step leaves the block.
+// holding it is one of at most four candidates from the slab's uniform
+// neighbour list; the lane picks by whether its y / z step leaves the block.
+// The slot is in bounds for every lane (a missing neighbour is the empty
+// brick), so lanes whose link J is a wall or iolet read it too and the
+// boundary delegates replace the value.
 template <int J>
 __device__ __forceinline__ uint32_t upstream_slot(const Lane& L) {
   constexpr int ox = -CX[J], oy = -CY[J], oz = -CZ[J];
   const bool xo = ox != 0 && (unsigned)(L.x + ox) > 7u;  // wave-uniform
   const bool yo = oy != 0 && (unsigned)(L.y + oy) > 7u;
   const bool zo = oz != 0 && (unsigned)(L.z + oz) > 7u;
-  constexpr int ix = (ox + 1) * 9, iy = (oy + 1) * 3, iz = oz + 1;
-  // Row entries for (dx, dy, dz) with each component either 0 or the step.
-  const int32_t c00 = xo ? ldp(L.row, ix + 4) : ldp(L.row, 13);
+  constexpr int iy = (oy + 1) * 3, iz = oz + 1;
+  // Entries for (dy, dz) with each component either 0 or the step, from the
+  // adjacent x layer of bricks when the x step leaves the brick (constant
+  // indices only, so the list stays in scalar registers).
+  auto pick = [&](int k) { return xo ? L.nb[9 + k] : L.nb[k]; };
+  const int32_t c00 = pick(4);
   int32_t brick = c00;
   if constexpr (oy != 0 && oz != 0) {
-    const int32_t c10 = xo ? ldp(L.row, ix + iy + 1) : ldp(L.row, 9 + iy + 1);
-    const int32_t c01 = xo ? ldp(L.row, ix + 3 + iz) : ldp(L.row, 9 + 3 + iz);
-    const int32_t c11 = xo ? ldp(L.row, ix + iy + iz) : ldp(L.row, 9 + iy + iz);
+    const int32_t c10 = pick(iy + 1);
+    const int32_t c01 = pick(3 + iz);
+    const int32_t c11 = pick(iy + iz);
     brick = yo ? (zo ? c11 : c10) : (zo ? c01 : c00);
   } else if constexpr (oy != 0) {
-    const int32_t c10 = xo ? ldp(L.row, ix + iy + 1) : ldp(L.row, 9 + iy + 1);
+    const int32_t c10 = pick(iy + 1);
     brick = yo ? c10 : c00;
   } else if constexpr (oz != 0) {
-    const int32_t c01 = xo ? ldp(L.row, ix + 3 + iz) : ldp(L.row, 9 + 3 + iz);
+    const int32_t c01 = pick(3 + iz);
     brick = zo ? c01 : c00;
   }
   const uint32_t local = ((uint32_t)((L.x + ox) & 7) * 8u + (uint32_t)((L.y + oy) & 7)) * 8u + (uint32_t)((L.z + oz) & 7);

@@ -113,8 +113,8 @@ struct hlb_lattice {
   int64_t S = 0, NB = 0, SB = 0, NI = 0, SI = 0;
   hlb::DevBuf<double> A[2], F, G, ladd_s, rho, u, wss, io_n, io_dens, io_vmax, io_vmax2, feq_vals;
   hlb::DevBuf<uint32_t> bwall, biolet;
-  hlb::DevBuf<int32_t> bgi, bii, nbr, slab, bbase;
-  hlb::DevBuf<uint64_t> fmask, bmask;
+  hlb::DevBuf<int32_t> bgi, bii;
+  hlb::DevBuf<hlb::SlabRec> rec;  // slab records
   hlb::DevBuf<float> bq, bnrm, io_nf;
   hlb::DevBuf<int> flag;
   // Guo-Zheng-Shi walls (params.wall_bc == HLB_WALL_GZS).
@@ -175,7 +175,7 @@ struct hlb_lattice {
     return hlb::GzsTables{pmask.p, pbase.p, bgz.p, bgzbase.p, SP};
   }
   hlb::BrickTables tables() const {
-    return hlb::BrickTables{nbr.p, slab.p, fmask.p, bmask.p, bbase.p, nslab};
+    return hlb::BrickTables{rec.p, nslab};
   }
 };
 

@@ -116,13 +116,13 @@ __device__ __forceinline__ Planes member_planes(const Member32& m, int step) {
   return p;
 }
 
+// The upstream gather, as gather_up in the FP64 pass: wall and iolet links
+// read the upstream slot too and the boundary delegates replace the value.
 template <int J = 1>
-__device__ __forceinline__ void gather_up32(const float* in, int64_t S, const Lane& L, uint32_t lm, float* g) {
+__device__ __forceinline__ void gather_up32(const float* in, int64_t S, const Lane& L, float* g) {
   if constexpr (J < Q) {
-    const uint32_t up = upstream_slot<J>(L);
-    const uint32_t src = (lm >> (INV[J] - 1)) & 1u ? L.slot : up;
-    g[J] = ldp(plane(in, S, J), src);
-    gather_up32<J + 1>(in, S, L, lm, g);
+    g[J] = ldp(plane(in, S, J), upstream_slot<J>(L));
+    gather_up32<J + 1>(in, S, L, g);
   }
 }
 
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1
     for (int j = 0; j < Q; ++j) g[j] = ldp(plane(P.in, m.S, j), d);
   } else {
     g[0] = ldp(P.in, d);
-    gather_up32(P.in, m.S, L, wm | im, g);
+    gather_up32(P.in, m.S, L, g);
     if (isB) boundary_fixup32(m, P.in, d, L.b, wm, im, g);
   }
   if (m.gz.pmask != nullptr) {
@@ -422,7 +422,7 @@ __global__ __launch_bounds__(kBlock) void gather_fp32(const Member32* __restrict
   }
   float g[Q];
   g[0] = ldp(in, L.slot);
-  gather_up32(in, m.S, L, wm | im, g);
+  gather_up32(in, m.S, L, g);
   if (L.boundary) boundary_fixup32(m, in, L.slot, L.b, wm, im, g);
 #pragma unroll
   for (int j = 0; j < Q; ++j) stp(plane(F, m.S, j), L.slot, g[j]);

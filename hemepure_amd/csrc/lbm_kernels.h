@@ -40,14 +40,28 @@ namespace hlb {
 constexpr int kBrickSites = 512;  // slots per brick (8^3)
 constexpr int kSlab = 64;         // slots per slab = lanes per wavefront
 
+// One active slab (64 slots: one x layer of a brick), everything a wave needs
+// before its first gather in one 128-byte record, so it costs one round of
+// scalar loads: the brick and x layer, the fluid and boundary lane masks, the
+// compact boundary index of its first boundary lane, and the bricks its pulls
+// can reach -- the 3 x 3 neighbours in its own x layer (dy, dz in -1..1, index
+// (dy+1)*3 + dz+1), then the same for the adjacent x layer of bricks its x
+// face pulls from (x = 0: dx = -1; x = 7: dx = +1; otherwise unused).  A
+// missing neighbour is the empty brick.
+struct SlabRec {
+  int32_t sid;      // brick * 8 + x
+  int32_t bbase;
+  uint64_t fmask;   // fluid lanes
+  uint64_t bmask;   // boundary lanes
+  int32_t nb[18];
+  int32_t pad[8];
+};
+static_assert(sizeof(SlabRec) == 128, "slab record is two 64-byte scalar loads");
+
 // The brick tables of one lattice.
 struct BrickTables {
-  const int32_t* nbr;    // (bricks + 1) x 27
-  const int32_t* slab;   // per active slab: brick * 8 + x
-  const uint64_t* fmask; // fluid lanes
-  const uint64_t* bmask; // boundary lanes
-  const int32_t* bbase;  // compact boundary index of the slab's first boundary lane
-  int64_t nslab;         // active slabs
+  const SlabRec* rec;  // per active slab
+  int64_t nslab;       // active slabs
 };
 constexpr int kMaxIolets = 256;  // iolets per lattice
 
