@@ -32,21 +32,24 @@ namespace hlb {
 
 namespace {
 
-// Gathers the upstream post-collision values.  Directions whose link ends
-// on a wall or iolet read whatever lies upstream (another brick's slot or
-// the empty brick's) without waiting for the boundary masks; the boundary
-// delegates replace those values.
+// Gathers the upstream post-collision values.  lm marks the directions
+// whose link ends on a wall or iolet (boundary lanes): those read the lane's
+// own slot, a line the wave fetches anyway, and the boundary delegates
+// replace the value.  (The FP32 pass reads upstream for those too, without
+// waiting for the masks; in FP64 the extra lines cost more than the wait.)
 template <int J = 1>
-__device__ __forceinline__ void gather_up(const double* const* ain, const Lane& L, double* f) {
+__device__ __forceinline__ void gather_up(const double* const* ain, const Lane& L, uint32_t lm, double* f) {
   if constexpr (J < Q) {
-    f[J] = ldp(ain[J], upstream_slot<J>(L));
-    gather_up<J + 1>(ain, L, f);
+    const uint32_t up = upstream_slot<J>(L);
+    const uint32_t src = (lm >> (INV[J] - 1)) & 1u ? L.slot : up;
+    f[J] = ldp(ain[J], src);
+    gather_up<J + 1>(ain, L, lm, f);
   }
 }
 
-__device__ __forceinline__ void gather_site(const StepArgs& a, const Lane& L, double* f) {
+__device__ __forceinline__ void gather_site(const StepArgs& a, const Lane& L, uint32_t wm, uint32_t im, double* f) {
   f[0] = ldp(a.ain[0], L.slot);
-  gather_up(a.ain, L, f);
+  gather_up(a.ain, L, wm | im, f);
 }
 
 // Boundary sites: per direction j the writer of the value is the site's link
@@ -245,7 +248,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1
 #pragma unroll
     for (int j = 0; j < Q; ++j) f[j] = ldp(a.ain[j], d);
   } else {
-    gather_site(a, L, f);
+    gather_site(a, L, wm, im, f);
     if (isB) boundary_fixup(a, d, L.b, wm, im, f);
   }
   if (a.gz.pmask != nullptr) {
@@ -308,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void gather_fp64(StepArgs a, double* __rest
     im = ldp(a.biolet, L.b);
   }
   double f[Q];
-  gather_site(a, L, f);
+  gather_site(a, L, wm, im, f);
   if (L.boundary) boundary_fixup(a, L.slot, L.b, wm, im, f);
 #pragma unroll
   for (int j = 0; j < Q; ++j) stp(plane(F, S, j), L.slot, f[j]);
