@@ -14,7 +14,7 @@ under "forms".
 Prints one JSON line (rank 0, N = 1).  MFLUPS = sites x steps / seconds / 1e6.
 The roofline object prices the step kernel against MI355X's 8 TB/s HBM with
 the algorithmic byte count per site update (FP64: 19 reads + 19 writes of 8 B
-= 304 B; the brick layout finds neighbours arithmetically, so no index table
+= 304 B; the slab layout finds neighbours arithmetically, so no index table
 is read).  cpu_baseline times the CPU oracle (the
 reference's push/AoS FP64 algorithm, OpenMP over site ranges) on a bounded
 sample of the same workload.
@@ -459,7 +459,7 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (procedurally generated seeded geometry)",
-        "config": {"workload": desc, "sites": n, "storage": "fp64 SoA in 8^3 bricks, arithmetic neighbour addressing",
+        "config": {"workload": desc, "sites": n, "storage": "fp64 SoA, compact 8x8 slabs of 8^3 bricks, arithmetic neighbour addressing",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),

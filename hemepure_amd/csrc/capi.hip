@@ -194,20 +194,22 @@ int side_iolet(const hlb_lattice* h, int64_t k) {
   return (c == 2 || c == 4) ? h->L.ioletId[k] : h->n_in + h->L.ioletId[k];
 }
 
-// The brick layout: the site grid cut into 8^3 tiles (independent of the
+// The slab layout: the site grid cut into 8^3 tiles (independent of the
 // geometry's own block size), occupied tiles numbered in tile order (z
-// fastest) as bricks; slots, the neighbour rows, the active slabs and the
-// boundary / iolet lists in slot order.
+// fastest) as bricks, each brick's x layers with a fluid site as slabs.  A
+// slab's fluid lanes (lane = y * 8 + z) take consecutive slots from its base;
+// bases are rounded up to kSlabAlign.  Boundary and iolet sites are numbered
+// in slot order.  Each slab's record lists the masks and bases of the slabs
+// its lanes pull from.
 void device_order(hlb_lattice* h) {
   const HostLattice& L = h->L;
   const int64_t N = L.N;
   const int64_t ntx = ((int64_t)L.blocks[0] * L.bs + 7) / 8, nty = ((int64_t)L.blocks[1] * L.bs + 7) / 8,
                 ntz = ((int64_t)L.blocks[2] * L.bs + 7) / 8;
+  auto tile_of = [&](int64_t x, int64_t y, int64_t z) { return (x * nty + y) * ntz + z; };
   std::vector<int32_t> brickOf((size_t)(ntx * nty * ntz), -1);
-  for (int64_t k = 0; k < N; ++k) {
-    const int64_t t = ((int64_t)(L.coords[3 * k] >> 3) * nty + (L.coords[3 * k + 1] >> 3)) * ntz + (L.coords[3 * k + 2] >> 3);
-    brickOf[t] = 0;
-  }
+  for (int64_t k = 0; k < N; ++k)
+    brickOf[tile_of(L.coords[3 * k] >> 3, L.coords[3 * k + 1] >> 3, L.coords[3 * k + 2] >> 3)] = 0;
   std::vector<int64_t> brickTile;
   for (size_t t = 0; t < brickOf.size(); ++t)
     if (brickOf[t] >= 0) {
@@ -216,55 +218,79 @@ void device_order(hlb_lattice* h) {
     }
   const int64_t NBr = (int64_t)brickTile.size();
   h->nbrick = NBr;
-  h->S = (NBr + 1) * kBrickSites;
-  h->perm.assign((size_t)h->S, -1);
-  h->inv.assign((size_t)N, -1);
+  // the site at each position of each brick
+  std::vector<int32_t> at((size_t)NBr * kBrickSites, -1);
   for (int64_t k = 0; k < N; ++k) {
     const int32_t x = L.coords[3 * k], y = L.coords[3 * k + 1], z = L.coords[3 * k + 2];
-    const int64_t brick = brickOf[((int64_t)(x >> 3) * nty + (y >> 3)) * ntz + (z >> 3)];
-    const int64_t slot = brick * kBrickSites + ((x & 7) * 8 + (y & 7)) * 8 + (z & 7);
-    h->perm[slot] = (int32_t)k;
-    h->inv[k] = (int32_t)slot;
+    const int64_t brick = brickOf[tile_of(x >> 3, y >> 3, z >> 3)];
+    at[(size_t)brick * kBrickSites + ((x & 7) * 8 + (y & 7)) * 8 + (z & 7)] = (int32_t)k;
   }
-  h->h_nbr.assign((size_t)(NBr + 1) * 27, (int32_t)NBr);
+  // active slabs, their masks and compact bases
+  std::vector<int32_t> slabOf((size_t)NBr * 8, -1);
   h->h_slab.clear();
+  h->h_sbase.clear();
   h->h_fmask.clear();
-  h->h_bmask.clear();
-  h->h_bbase.clear();
+  int64_t next = 0;
+  for (int64_t k = 0; k < NBr; ++k)
+    for (int x = 0; x < 8; ++x) {
+      uint64_t fm = 0;
+      for (int lane = 0; lane < kSlab; ++lane)
+        if (at[(size_t)k * kBrickSites + x * kSlab + lane] >= 0) fm |= 1ull << lane;
+      if (fm == 0) continue;
+      next = round_up(next, kSlabAlign);
+      slabOf[(size_t)k * 8 + x] = (int32_t)h->h_slab.size();
+      h->h_slab.push_back((int32_t)(k * 8 + x));
+      h->h_sbase.push_back((int32_t)next);
+      h->h_fmask.push_back(fm);
+      next += __builtin_popcountll(fm);
+    }
+  h->nslab = (int64_t)h->h_slab.size();
+  h->S = round_up(std::max<int64_t>(next, 1), 64);
+  h->perm.assign((size_t)h->S, -1);
+  h->inv.assign((size_t)N, -1);
   h->bsite.clear();
   h->isite.clear();
-  for (int64_t k = 0; k < NBr; ++k) {
+  h->h_rec.assign((size_t)std::max<int64_t>(h->nslab, 1), SlabRec{});
+  // the slab at layer x + dx of the brick at (dy, dz) from brick k, or -1
+  auto reach = [&](int64_t k, int x, int dx, int dy, int dz) -> int32_t {
+    const int xx = x + dx;
+    const int bx = xx < 0 ? -1 : (xx > 7 ? 1 : 0);
     const int64_t t = brickTile[k];
-    const int64_t tx = t / (nty * ntz), ty = (t / ntz) % nty, tz = t % ntz;
+    const int64_t tx = t / (nty * ntz) + bx, ty = (t / ntz) % nty + dy, tz = t % ntz + dz;
+    if (tx < 0 || ty < 0 || tz < 0 || tx >= ntx || ty >= nty || tz >= ntz) return -1;
+    const int32_t nb = brickOf[tile_of(tx, ty, tz)];
+    return nb < 0 ? -1 : slabOf[(size_t)nb * 8 + (xx & 7)];
+  };
+  for (int64_t r = 0; r < h->nslab; ++r) {
+    const int64_t k = h->h_slab[r] >> 3;
+    const int x = h->h_slab[r] & 7;
+    SlabRec& R = h->h_rec[r];
+    std::memset(&R, 0, sizeof R);
+    R.sid = h->h_slab[r];
+    R.bbase = (int32_t)h->bsite.size();
     for (int dx = -1; dx <= 1; ++dx)
       for (int dy = -1; dy <= 1; ++dy)
         for (int dz = -1; dz <= 1; ++dz) {
-          const int64_t x = tx + dx, y = ty + dy, z = tz + dz;
-          if (x < 0 || y < 0 || z < 0 || x >= ntx || y >= nty || z >= ntz) continue;
-          const int32_t nb = brickOf[(x * nty + y) * ntz + z];
-          if (nb >= 0) h->h_nbr[(size_t)k * 27 + (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)] = nb;
+          if (dx != 0 && dy != 0 && dz != 0) continue;  // no such link in D3Q19
+          const int e = reach_entry(dx, dy, dz);
+          const int32_t q = reach(k, x, dx, dy, dz);
+          R.fm[e] = q < 0 ? 0 : h->h_fmask[q];
+          R.base[e] = q < 0 ? 0 : h->h_sbase[q];
         }
-    for (int x = 0; x < 8; ++x) {
-      uint64_t fm = 0, bm = 0;
-      const int32_t base = (int32_t)h->bsite.size();
-      for (int lane = 0; lane < kSlab; ++lane) {
-        const int32_t site = h->perm[k * kBrickSites + x * kSlab + lane];
-        if (site < 0) continue;
-        fm |= 1ull << lane;
-        if (site >= L.off[1]) {
-          bm |= 1ull << lane;
-          h->bsite.push_back(site);
-        }
-        if (site >= L.off[2]) h->isite.push_back(site);
+    uint32_t slot = (uint32_t)h->h_sbase[r];
+    for (int lane = 0; lane < kSlab; ++lane) {
+      const int32_t site = at[(size_t)k * kBrickSites + x * kSlab + lane];
+      if (site < 0) continue;
+      h->perm[slot] = site;
+      h->inv[site] = (int32_t)slot;
+      ++slot;
+      if (site >= L.off[1]) {
+        R.bmask |= 1ull << lane;
+        h->bsite.push_back(site);
       }
-      if (fm == 0) continue;
-      h->h_slab.push_back((int32_t)(k * 8 + x));
-      h->h_fmask.push_back(fm);
-      h->h_bmask.push_back(bm);
-      h->h_bbase.push_back(base);
+      if (site >= L.off[2]) h->isite.push_back(site);
     }
   }
-  h->nslab = (int64_t)h->h_slab.size();
   h->NB = (int64_t)h->bsite.size();
   h->NI = (int64_t)h->isite.size();
   h->SB = round_up(std::max<int64_t>(h->NB, 1), 64);
@@ -416,12 +442,14 @@ int gzs_setup(hlb_lattice* h) {
   }
   std::vector<int32_t> pidx(L.N, -1);
   for (size_t r = 0; r < h->h_slab.size(); ++r) {
-    const int64_t base = (int64_t)(h->h_slab[r] >> 3) * kBrickSites + (int64_t)(h->h_slab[r] & 7) * kSlab;
+    const uint64_t fm = h->h_fmask[r];
     uint64_t pm = 0;
     h->h_pbase.push_back((int32_t)h->NP);
+    int32_t slot = h->h_sbase[r];
     for (int lane = 0; lane < kSlab; ++lane) {
-      const int32_t k = h->perm[base + lane];
-      if (k < 0 || !needP[k]) continue;
+      if (!((fm >> lane) & 1u)) continue;
+      const int32_t k = h->perm[slot++];
+      if (!needP[k]) continue;
       pm |= 1ull << lane;
       pidx[k] = (int32_t)h->NP++;
     }
@@ -496,29 +524,9 @@ int upload_lattice(hlb_lattice* h) {
       }
     }
   }
-  // Slab records: masks, boundary base and the reachable bricks.
-  {
-    const size_t ns = h->h_slab.size();
-    const int32_t empty = (int32_t)h->nbrick;
-    std::vector<SlabRec> recs(std::max<size_t>(1, ns));
-    for (size_t r = 0; r < ns; ++r) {
-      SlabRec& R = recs[r];
-      std::memset(&R, 0, sizeof R);
-      R.sid = h->h_slab[r];
-      R.bbase = h->h_bbase[r];
-      R.fmask = h->h_fmask[r];
-      R.bmask = h->h_bmask[r];
-      const int64_t brick = R.sid >> 3;
-      const int x = R.sid & 7;
-      const int dxo = x == 0 ? -1 : (x == 7 ? 1 : 0);
-      for (int k = 0; k < 9; ++k) {
-        R.nb[k] = h->h_nbr[(size_t)brick * 27 + 9 + k];
-        R.nb[9 + k] = dxo ? h->h_nbr[(size_t)brick * 27 + (dxo + 1) * 9 + k] : empty;
-      }
-    }
-    HLB_CHECK(h->rec.alloc(recs.size()));
-    HLB_CHECK(hipMemcpy(h->rec.p, recs.data(), sizeof(SlabRec) * recs.size(), hipMemcpyHostToDevice));
-  }
+  // Slab records.
+  HLB_CHECK(h->rec.alloc(h->h_rec.size()));
+  HLB_CHECK(hipMemcpy(h->rec.p, h->h_rec.data(), sizeof(SlabRec) * h->h_rec.size(), hipMemcpyHostToDevice));
   // Boundary arrays in device order.
   {
     std::vector<int32_t> iolIndex(N, -1);
@@ -1261,12 +1269,12 @@ extern "C" int hlb_neighbour_tables(hlb_handle h, int64_t* push, uint32_t* pull)
     std::memcpy(pull, h->L.pull.data(), sizeof(uint32_t) * h->L.pull.size());
   } else if (pull) {
     // The slots the device kernels gather from, computed by the same code
-    // on the device, expressed in the reference's order (a hole reads as
+    // on the device, expressed in the reference's order (a padding slot reads as
     // 0xffffffff).
     const int64_t N = h->L.N, S = h->S;
     DevBuf<uint32_t> dev;
     HLB_CHECK(dev.alloc((size_t)18 * S));
-    HLB_CHECK(launch_upstream_table(h->tables(), h->bwall.p, h->biolet.p, dev.p, S, h->stream));
+    HLB_CHECK(launch_upstream_table(h->tables(), dev.p, S, h->stream));
     std::vector<uint32_t> tmp((size_t)18 * S);
     HLB_CHECK(hipMemcpyAsync(tmp.data(), dev.p, sizeof(uint32_t) * tmp.size(), hipMemcpyDeviceToHost, h->stream));
     HLB_CHECK(hipStreamSynchronize(h->stream));
@@ -1307,7 +1315,7 @@ int64_t device_bytes(const hlb_lattice* h) {
   const int64_t S = h->S, SB = h->SB, SI = h->SI;
   const int64_t fsz = h->fp32() ? 4 : 8;
   int64_t b = 3 * Q * fsz * S + Q * fsz * SI;        // A[2], F, G
-  b += (int64_t)sizeof(SlabRec) * (S / kSlab);       // slab records (at most one per 64 slots)
+  b += (int64_t)sizeof(SlabRec) * std::max<int64_t>(h->nslab, 1);  // slab records
   b += (4 * 4 + 18 * 4 + 3 * 4) * SB;                 // boundary masks, ids, q, normals
   b += 36 * 8 * SI + 8 * SI;                          // Ladd factors, Windkessel terms
   b += 5 * 8 * S;                                     // rho, u, wss

@@ -57,42 +57,45 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// One lane of a slab: its slot, block-local coordinates and boundary index.
+// One lane of a slab: its slot, block-local (y, z) and boundary index.
 struct Lane {
-  uint32_t r;        // slab record (wave-uniform)
+  uint32_t r;              // slab record (wave-uniform)
   uint32_t slot;
-  int x, y, z;       // x is wave-uniform
+  int y, z;
   bool boundary;
-  uint32_t b;        // compact boundary index (valid when boundary)
-  int32_t nb[18];    // the slab's reachable bricks (wave-uniform), as in SlabRec
+  uint32_t b;              // compact boundary index (valid when boundary)
+  uint64_t fm[kReach];     // reachable slabs' fluid masks (wave-uniform)
+  int32_t base[kReach];    // and their first slots
 };
 
-// Decodes slab record r for this lane; false for a hole or a record past the
-// end (the lane has nothing to do).  r must be wave-uniform; the record is
-// read with scalar loads that do not wait on one another.
+// Number of set bits of the wave-uniform mask below this lane, plus add.
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t mask, uint32_t add) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, add));
+}
+
+// Decodes slab record r for this lane; false for a position that is not
+// fluid or a record past the end (the lane has nothing to do).  r must be
+// wave-uniform; the record is read with scalar loads that do not wait on one
+// another.
 __device__ __forceinline__ bool slab_lane(const BrickTables& bt, uint32_t r, Lane& L) {
   if (r >= (uint64_t)bt.nslab) return false;
   const uint32_t lane = threadIdx.x & 63u;
   const HLB_GLOBAL SlabRec* g = (const HLB_GLOBAL SlabRec*)(bt.rec + r);
-  const uint64_t fm = uniform64(g->fmask);
-  const uint64_t bm = uniform64(g->bmask);
-  const int32_t sid = __builtin_amdgcn_readfirstlane(g->sid);
-  const int32_t bbase = __builtin_amdgcn_readfirstlane(g->bbase);
 #pragma unroll
-  for (int k = 0; k < 18; ++k) L.nb[k] = __builtin_amdgcn_readfirstlane(g->nb[k]);
+  for (int k = 0; k < kReach; ++k) {
+    L.fm[k] = uniform64(g->fm[k]);
+    L.base[k] = __builtin_amdgcn_readfirstlane(g->base[k]);
+  }
+  const uint64_t bm = uniform64(g->bmask);
+  const int32_t bbase = __builtin_amdgcn_readfirstlane(g->bbase);
+  const uint64_t fm = L.fm[reach_entry(0, 0, 0)];
   if (!((fm >> lane) & 1u)) return false;
-  const uint32_t brick = (uint32_t)sid >> 3;
   L.r = r;
-  L.x = sid & 7;
   L.y = (int)(lane >> 3);
   L.z = (int)(lane & 7u);
-  L.slot = brick * (uint32_t)kBrickSites + (uint32_t)L.x * 64u + lane;
+  L.slot = mbcnt64(fm, (uint32_t)L.base[reach_entry(0, 0, 0)]);
   L.boundary = (bm >> lane) & 1u;
-  L.b = 0;
-  if (L.boundary) {
-    const uint64_t below = bm & ((1ull << lane) - 1ull);
-    L.b = (uint32_t)bbase + (uint32_t)__builtin_popcountll(below);
-  }
+  L.b = L.boundary ? mbcnt64(bm, (uint32_t)bbase) : 0u;
   return true;
 }
 
@@ -102,39 +105,48 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
   return (uint32_t)__builtin_popcountll(mask & ((1ull << lane) - 1ull));
 }
 
-// Slot of the upstream site x - c_J of a lane (pull direction J).  The brick
-// holding it is one of at most four candidates from the slab's uniform
-// neighbour list; the lane picks by whether its y / z step leaves the block.
-// The slot is in bounds for every lane (a missing neighbour is the empty
-// brick), so lanes whose link J is a wall or iolet read it too and the
-// boundary delegates replace the value.
+// Slot of the upstream site x - c_J of a lane (pull direction J), or the
+// lane's own slot when that position is not fluid (the link is a wall or
+// iolet link; its boundary delegate replaces the value).  The slab holding
+// the position is one of at most four candidates from the record; the lane
+// picks by whether its y / z step leaves the brick, then counts the fluid
+// lanes below the position in that slab.
 template <int J>
 __device__ __forceinline__ uint32_t upstream_slot(const Lane& L) {
   constexpr int ox = -CX[J], oy = -CY[J], oz = -CZ[J];
-  const bool xo = ox != 0 && (unsigned)(L.x + ox) > 7u;  // wave-uniform
-  const bool yo = oy != 0 && (unsigned)(L.y + oy) > 7u;
-  const bool zo = oz != 0 && (unsigned)(L.z + oz) > 7u;
-  constexpr int iy = (oy + 1) * 3, iz = oz + 1;
-  // Entries for (dy, dz) with each component either 0 or the step, from the
-  // adjacent x layer of bricks when the x step leaves the brick (constant
-  // indices only, so the list stays in scalar registers).
-  auto pick = [&](int k) { return xo ? L.nb[9 + k] : L.nb[k]; };
-  const int32_t c00 = pick(4);
-  int32_t brick = c00;
+  const int yy = L.y + oy, zz = L.z + oz;
+  const bool yo = oy != 0 && (unsigned)yy > 7u;
+  const bool zo = oz != 0 && (unsigned)zz > 7u;
+  // constant entry indices only, so the masks and bases stay in scalar registers
+  // (values are copied out before selecting: a select between array elements
+  // would select their addresses and keep the record in scratch memory)
+  constexpr int e00 = reach_entry(ox, 0, 0);
+  uint64_t m = L.fm[e00];
+  int32_t base = L.base[e00];
   if constexpr (oy != 0 && oz != 0) {
-    const int32_t c10 = pick(iy + 1);
-    const int32_t c01 = pick(3 + iz);
-    const int32_t c11 = pick(iy + iz);
-    brick = yo ? (zo ? c11 : c10) : (zo ? c01 : c00);
+    static_assert(ox == 0, "D3Q19 links have at most two nonzero components");
+    constexpr int e10 = reach_entry(0, oy, 0), e01 = reach_entry(0, 0, oz), e11 = reach_entry(0, oy, oz);
+    const uint64_t m10 = L.fm[e10], m01 = L.fm[e01], m11 = L.fm[e11];
+    const int32_t b10 = L.base[e10], b01 = L.base[e01], b11 = L.base[e11];
+    m = yo ? (zo ? m11 : m10) : (zo ? m01 : m);
+    base = yo ? (zo ? b11 : b10) : (zo ? b01 : base);
   } else if constexpr (oy != 0) {
-    const int32_t c10 = pick(iy + 1);
-    brick = yo ? c10 : c00;
+    constexpr int e10 = reach_entry(ox, oy, 0);
+    const uint64_t m10 = L.fm[e10];
+    const int32_t b10 = L.base[e10];
+    m = yo ? m10 : m;
+    base = yo ? b10 : base;
   } else if constexpr (oz != 0) {
-    const int32_t c01 = pick(3 + iz);
-    brick = zo ? c01 : c00;
+    constexpr int e01 = reach_entry(ox, 0, oz);
+    const uint64_t m01 = L.fm[e01];
+    const int32_t b01 = L.base[e01];
+    m = zo ? m01 : m;
+    base = zo ? b01 : base;
   }
-  const uint32_t local = ((uint32_t)((L.x + ox) & 7) * 8u + (uint32_t)((L.y + oy) & 7)) * 8u + (uint32_t)((L.z + oz) & 7);
-  return (uint32_t)brick * (uint32_t)kBrickSites + local;
+  const uint32_t pos = ((uint32_t)(yy & 7) << 3) | (uint32_t)(zz & 7);
+  const uint64_t bit = 1ull << pos;
+  const uint32_t slot = (uint32_t)base + (uint32_t)__builtin_popcountll(m & (bit - 1ull));
+  return (m & bit) ? slot : L.slot;
 }
 
 }  // namespace hlb

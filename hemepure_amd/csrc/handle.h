@@ -19,8 +19,7 @@ hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool prop
 hipError_t launch_gzs_fp64(const StepArgs& a, int kernel, const GzsLink* links, int n, const double* vmax,
                            double* GW, hipStream_t s);
 hipError_t launch_gzs_fp32(const Member32* members, const GzsLink* links, int n, int step, hipStream_t s);
-hipError_t launch_upstream_table(const BrickTables& bt, const uint32_t* bwall, const uint32_t* biolet, uint32_t* out,
-                                 int64_t S, hipStream_t s);
+hipError_t launch_upstream_table(const BrickTables& bt, uint32_t* out, int64_t S, hipStream_t s);
 size_t monitor_scratch(int64_t nslab);
 hipError_t launch_monitor(const BrickTables& bt, const double* rho, const double* u, int64_t S, void* scratch,
                           double* out, hipStream_t s);
@@ -101,12 +100,13 @@ struct hlb_lattice {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // Device slots (brick layout): perm[slot] = reference index or -1 for a
-  // hole, inv[k] = slot of reference index k.
+  // Device slots (compact slab layout): perm[slot] = reference index or -1
+  // for padding, inv[k] = slot of reference index k.
   std::vector<int32_t> perm, inv;
   int64_t nbrick = 0, nslab = 0;
-  std::vector<int32_t> h_nbr, h_slab, h_bbase;  // brick tables (host copies)
-  std::vector<uint64_t> h_fmask, h_bmask;
+  std::vector<int32_t> h_slab, h_sbase;  // per active slab: brick * 8 + x, first slot
+  std::vector<uint64_t> h_fmask;          // per active slab: fluid lanes
+  std::vector<hlb::SlabRec> h_rec;        // the records the kernels read
   // Boundary sites (classes 1-5) and iolet sites in device order.
   std::vector<int32_t> bsite;  // reference index of boundary slot b
   std::vector<int32_t> isite;  // reference index of iolet slot i

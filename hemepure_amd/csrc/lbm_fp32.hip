@@ -117,7 +117,7 @@ __device__ __forceinline__ Planes member_planes(const Member32& m, int step) {
 }
 
 // The upstream gather, as gather_up in the FP64 pass: wall and iolet links
-// read the upstream slot too and the boundary delegates replace the value.
+// read the lane's own slot and the boundary delegates replace the value.
 template <int J = 1>
 __device__ __forceinline__ void gather_up32(const float* in, int64_t S, const Lane& L, float* g) {
   if constexpr (J < Q) {

@@ -32,24 +32,22 @@ namespace hlb {
 
 namespace {
 
-// Gathers the upstream post-collision values.  lm marks the directions
-// whose link ends on a wall or iolet (boundary lanes): those read the lane's
-// own slot, a line the wave fetches anyway, and the boundary delegates
-// replace the value.  (The FP32 pass reads upstream for those too, without
-// waiting for the masks; in FP64 the extra lines cost more than the wait.)
+// Gathers the upstream post-collision values.  A direction whose upstream
+// position is not fluid (a wall or iolet link) reads the lane's own slot, a
+// line the wave fetches anyway, and the boundary delegates replace the value;
+// the choice needs only the slab record, so the gathers do not wait on the
+// boundary masks.
 template <int J = 1>
-__device__ __forceinline__ void gather_up(const double* const* ain, const Lane& L, uint32_t lm, double* f) {
+__device__ __forceinline__ void gather_up(const double* const* ain, const Lane& L, double* f) {
   if constexpr (J < Q) {
-    const uint32_t up = upstream_slot<J>(L);
-    const uint32_t src = (lm >> (INV[J] - 1)) & 1u ? L.slot : up;
-    f[J] = ldp(ain[J], src);
-    gather_up<J + 1>(ain, L, lm, f);
+    f[J] = ldp(ain[J], upstream_slot<J>(L));
+    gather_up<J + 1>(ain, L, f);
   }
 }
 
-__device__ __forceinline__ void gather_site(const StepArgs& a, const Lane& L, uint32_t wm, uint32_t im, double* f) {
+__device__ __forceinline__ void gather_site(const StepArgs& a, const Lane& L, double* f) {
   f[0] = ldp(a.ain[0], L.slot);
-  gather_up(a.ain, L, wm | im, f);
+  gather_up(a.ain, L, f);
 }
 
 // Boundary sites: per direction j the writer of the value is the site's link
@@ -248,7 +246,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1
 #pragma unroll
     for (int j = 0; j < Q; ++j) f[j] = ldp(a.ain[j], d);
   } else {
-    gather_site(a, L, wm, im, f);
+    gather_site(a, L, f);
     if (isB) boundary_fixup(a, d, L.b, wm, im, f);
   }
   if (a.gz.pmask != nullptr) {
@@ -311,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void gather_fp64(StepArgs a, double* __rest
     im = ldp(a.biolet, L.b);
   }
   double f[Q];
-  gather_site(a, L, wm, im, f);
+  gather_site(a, L, f);
   if (L.boundary) boundary_fixup(a, L.slot, L.b, wm, im, f);
 #pragma unroll
   for (int j = 0; j < Q; ++j) stp(plane(F, S, j), L.slot, f[j]);
@@ -471,21 +469,17 @@ __global__ __launch_bounds__(kBlock) void gzs_links_fp64(StepArgs a, const GzsLi
 // wall or iolet link), 18 planes of S: the step kernels' addressing, exported
 // so the host can compare it with the reference's neighbour table.
 template <int J = 1>
-__device__ __forceinline__ void upstream_all(const Lane& L, uint32_t lm, uint32_t* out, int64_t S) {
+__device__ __forceinline__ void upstream_all(const Lane& L, uint32_t* out, int64_t S) {
   if constexpr (J < Q) {
-    const uint32_t up = upstream_slot<J>(L);
-    stp(plane(out, S, J - 1), L.slot, (lm >> (INV[J] - 1)) & 1u ? L.slot : up);
-    upstream_all<J + 1>(L, lm, out, S);
+    stp(plane(out, S, J - 1), L.slot, upstream_slot<J>(L));
+    upstream_all<J + 1>(L, out, S);
   }
 }
 
-__global__ __launch_bounds__(kBlock) void upstream_table(BrickTables bt, const uint32_t* __restrict__ bwall,
-                                                         const uint32_t* __restrict__ biolet,
-                                                         uint32_t* __restrict__ out, int64_t S) {
+__global__ __launch_bounds__(kBlock) void upstream_table(BrickTables bt, uint32_t* __restrict__ out, int64_t S) {
   Lane L;
   if (!slab_lane(bt, slab_record(), L)) return;
-  const uint32_t lm = L.boundary ? (ldp(bwall, L.b) | ldp(biolet, L.b)) : 0u;
-  upstream_all(L, lm, out, S);
+  upstream_all(L, out, S);
 }
 
 // Monitors over the fluid sites' property fields (the last step that filled
@@ -724,10 +718,9 @@ hipError_t launch_gzs_fp64(const StepArgs& a, int kernel, const GzsLink* links, 
   return hipGetLastError();
 }
 
-hipError_t launch_upstream_table(const BrickTables& bt, const uint32_t* bwall, const uint32_t* biolet, uint32_t* out,
-                                 int64_t S, hipStream_t s) {
+hipError_t launch_upstream_table(const BrickTables& bt, uint32_t* out, int64_t S, hipStream_t s) {
   if (bt.nslab <= 0) return hipSuccess;
-  hipLaunchKernelGGL(upstream_table, dim3(slab_grid(bt.nslab)), dim3(kBlock), 0, s, bt, bwall, biolet, out, S);
+  hipLaunchKernelGGL(upstream_table, dim3(slab_grid(bt.nslab)), dim3(kBlock), 0, s, bt, out, S);
   return hipGetLastError();
 }
 

@@ -1,30 +1,31 @@
 // lbm_kernels.h -- device-side data layout shared by the kernels and the
 // host launcher.
 //
-// Brick layout.  The reference's LatticeData orders sites class-major
+// Slab layout.  The reference's LatticeData orders sites class-major
 // (LatticeData.h:375-436); the host keeps that order for every API array.  On
-// the device every occupied 8^3 block of the geometry is a brick of 512
-// slots, bricks in the reference's block order, slot = brick * 512 +
-// (x * 8 + y) * 8 + z with block-local coordinates (the reference's in-block
-// order, z fastest).  Solid positions are holes that no kernel writes.  One
-// wavefront handles one slab (the 64 slots of one x) and a launch walks the
-// list of slabs that hold at least one fluid site.  A site's upstream
-// neighbour is found arithmetically: the same brick, or one of its 26
-// neighbouring bricks from a 27-entry row per brick, so no per-site
-// neighbour table is read.  Links that end on a wall or iolet read the site's
-// own slot instead and the boundary delegates overwrite the value.
+// the device the site grid is cut into 8^3 bricks (occupied tiles in tile
+// order, z fastest) and each brick into 8 slabs, one per block-local x; a
+// slab's 64 lanes are its (y, z) positions, lane = y * 8 + z (the reference's
+// in-block order, z fastest).  One wavefront handles one slab and a launch
+// walks the list of slabs that hold at least one fluid site.  Storage is
+// compact: the fluid lanes of a slab occupy consecutive slots from the slab's
+// base in lane order (slot = base + number of fluid lanes below), bases
+// rounded up to kSlabAlign slots.  (Measured on MI355X: 32-slot bases keep
+// the read and write bytes at the layout's own size; 16-slot bases more
+// than doubled both in the step kernels.)  A
+// site's upstream neighbour is found arithmetically from the fluid masks and
+// bases of the slabs around it, which the slab's record lists, so no
+// per-site neighbour table is read.  A link whose upstream position is not
+// fluid (a wall or iolet link) reads the site's own slot instead, and the
+// boundary delegates overwrite the value.
 //
-// HBM layout (S = (bricks + 1) * 512 slots; the extra brick is the empty
-// neighbour of bricks at the edge of the geometry):
+// HBM layout (S = compact slots rounded up to 64):
 //   A[2]   post-collision distributions, 19 planes of S (double, or float
 //          g = f - w rho0 in the FP32-shifted form).
 //   F      pre-collision staging of the same shape (initial state, uploads,
 //          downloads).
-//   nbr    (bricks + 1) x 27 brick ids, index (dx+1)*9 + (dy+1)*3 + (dz+1).
-//   slabs  per active slab: brick * 8 + x, the fluid-lane and boundary-lane
-//          masks (lane = y * 8 + z) and the compact index of its first
-//          boundary lane; boundary sites (collision classes 1-5) are numbered
-//          in slot order.
+//   rec    one 256-byte record per active slab (SlabRec).  Boundary sites
+//          (collision classes 1-5) are numbered in slot order.
 //   boundary arrays (NB boundary sites, stride SB): wall and iolet link
 //          masks, global iolet index, compact iolet index, 18 planes of float
 //          link distances, 3 planes of float wall normals.
@@ -37,26 +38,36 @@
 
 namespace hlb {
 
-constexpr int kBrickSites = 512;  // slots per brick (8^3)
-constexpr int kSlab = 64;         // slots per slab = lanes per wavefront
+constexpr int kBrickSites = 512;  // positions per brick (8^3)
+constexpr int kSlab = 64;         // positions per slab = lanes per wavefront
+constexpr int kSlabAlign = 32;    // slab bases are multiples of this many slots
 
-// One active slab (64 slots: one x layer of a brick), everything a wave needs
-// before its first gather in one 128-byte record, so it costs one round of
-// scalar loads: the brick and x layer, the fluid and boundary lane masks, the
-// compact boundary index of its first boundary lane, and the bricks its pulls
-// can reach -- the 3 x 3 neighbours in its own x layer (dy, dz in -1..1, index
-// (dy+1)*3 + dz+1), then the same for the adjacent x layer of bricks its x
-// face pulls from (x = 0: dx = -1; x = 7: dx = +1; otherwise unused).  A
-// missing neighbour is the empty brick.
+// The slabs a lane can pull from: entries 0..8 are the slab's own x layer in
+// the 3 x 3 bricks around it (index (dy+1)*3 + dz+1, 4 = the slab itself);
+// 9..13 the layer at x - 1 and 14..18 the layer at x + 1 (in this brick or
+// the next one in x), each in the brick itself, at dy -1, dy +1, dz -1,
+// dz +1.  D3Q19 has no link with all three components nonzero, so no other
+// slab is ever read.
+constexpr int kReach = 19;
+constexpr int reach_entry(int dx, int dy, int dz) {
+  return dx == 0 ? (dy + 1) * 3 + (dz + 1)
+                 : (dx < 0 ? 9 : 14) + (dy == 0 ? (dz == 0 ? 0 : (dz < 0 ? 3 : 4)) : (dy < 0 ? 1 : 2));
+}
+
+// One active slab: everything a wave needs before its first gather, in one
+// 256-byte record read with scalar loads that do not wait on one another --
+// the fluid masks and compact bases of the reachable slabs (a missing or
+// empty slab has mask 0), the boundary-lane mask and the compact index of its
+// first boundary lane.
 struct SlabRec {
-  int32_t sid;      // brick * 8 + x
-  int32_t bbase;
-  uint64_t fmask;   // fluid lanes
-  uint64_t bmask;   // boundary lanes
-  int32_t nb[18];
-  int32_t pad[8];
+  uint64_t fm[kReach];    // fluid lanes of each reachable slab
+  uint64_t bmask;         // boundary lanes of this slab
+  int32_t base[kReach];   // first slot of each reachable slab
+  int32_t bbase;          // boundary index of this slab's first boundary lane
+  int32_t sid;            // brick * 8 + x (host bookkeeping)
+  int32_t pad[3];
 };
-static_assert(sizeof(SlabRec) == 128, "slab record is two 64-byte scalar loads");
+static_assert(sizeof(SlabRec) == 256, "slab record is four 64-byte scalar loads");
 
 // The brick tables of one lattice.
 struct BrickTables {

@@ -206,7 +206,7 @@ extern "C" int hlb_server_advance(hlb_server_handle s, int* running, int* queued
     for (size_t i = 0; i < at_cycle.size(); ++i) {
       Request* r = at_cycle[i];
       HLB_CHECK(hipMemsetAsync(s->d_delta, 0, sizeof(unsigned), st));
-      // every slot: holes hold u = 0 in both arrays after the first cycle end
+      // every slot: padding slots hold u = 0 in both arrays after the first cycle end
       HLB_CHECK(launch_cycle_delta(r->lat->u.p, r->prev, r->lat->S, r->lat->S, s->d_delta, st));
       HLB_CHECK(hipMemcpyAsync(&bits[i], s->d_delta, sizeof(unsigned), hipMemcpyDeviceToHost, st));
     }

@@ -222,7 +222,7 @@ def test_long_table_chunks():
 
 def test_device_addressing_matches_host_pull_table():
     """The slots the step kernels gather from, computed on the device from
-    the brick tables, equal the host pull table (and so the reference's push
+    the slab records, equal the host pull table (and so the reference's push
     table inverted) on every table geometry, whatever its block size."""
     from test_tables import _geoms, _handle
 
