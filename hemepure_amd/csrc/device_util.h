@@ -57,6 +57,14 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// Plane j of a wave-uniform array, made an opaque scalar value so every
+// access is (scalar base) + (32-bit lane offset) rather than a 64-bit vector
+// address the compiler rebuilds per plane.
+template <typename T>
+__device__ __forceinline__ T* uplane(T* p, int64_t S, int j) {
+  return reinterpret_cast<T*>(uniform64(reinterpret_cast<uint64_t>(p + (int64_t)j * S)));
+}
+
 // One lane of a slab: its slot, block-local (y, z) and boundary index.
 struct Lane {
   uint32_t r;              // slab record (wave-uniform)
@@ -105,48 +113,74 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
   return (uint32_t)__builtin_popcountll(mask & ((1ull << lane) - 1ull));
 }
 
+// Slot of position lane + DELTA of a slab with fluid mask m and first slot
+// base; fluid receives m shifted so that bit `lane` says whether that
+// position is fluid.  m and base are wave-uniform, so the shift and the count
+// of fluid positions below DELTA are scalar work and the per-lane count is
+// one mbcnt pair.
+template <int DELTA>
+__device__ __forceinline__ uint32_t shifted_slot(uint64_t m, int32_t base, uint64_t& fluid) {
+  uint64_t ms = m;
+  uint32_t add = (uint32_t)base;
+  if constexpr (DELTA > 0) {
+    ms = m >> DELTA;
+    add += (uint32_t)__builtin_popcountll(m & ((1ull << DELTA) - 1ull));
+  } else if constexpr (DELTA < 0) {
+    ms = m << (-DELTA);
+  }
+  fluid = ms;
+  return mbcnt64(ms, add);
+}
+
+// Lanes at a brick face: y = 0, y = 7, z = 0, z = 7 (lane = y * 8 + z).
+constexpr uint64_t kLanesY0 = 0x00000000000000ffull, kLanesY7 = 0xff00000000000000ull;
+constexpr uint64_t kLanesZ0 = 0x0101010101010101ull, kLanesZ7 = 0x8080808080808080ull;
+
 // Slot of the upstream site x - c_J of a lane (pull direction J), or the
 // lane's own slot when that position is not fluid (the link is a wall or
-// iolet link; its boundary delegate replaces the value).  The slab holding
-// the position is one of at most four candidates from the record; the lane
-// picks by whether its y / z step leaves the brick, then counts the fluid
-// lanes below the position in that slab.
+// iolet link; its boundary delegate replaces the value).  The position is
+// lane + 8 oy + oz in the slab the record names for the x step, or -- for
+// lanes whose y / z step leaves the brick -- 64 or 8 positions back in the
+// neighbouring brick's slab: at most four candidates, each one mbcnt pair,
+// picked with constant lane masks; the fluid test combines the candidates'
+// shifted masks with scalar logic into one lane mask.
 template <int J>
 __device__ __forceinline__ uint32_t upstream_slot(const Lane& L) {
   constexpr int ox = -CX[J], oy = -CY[J], oz = -CZ[J];
-  const int yy = L.y + oy, zz = L.z + oz;
-  const bool yo = oy != 0 && (unsigned)yy > 7u;
-  const bool zo = oz != 0 && (unsigned)zz > 7u;
+  constexpr int d00 = 8 * oy + oz;
+  // lanes whose step leaves the brick in y / in z
+  constexpr uint64_t Y = oy == 0 ? 0ull : (oy < 0 ? kLanesY0 : kLanesY7);
+  constexpr uint64_t Z = oz == 0 ? 0ull : (oz < 0 ? kLanesZ0 : kLanesZ7);
+  const bool yo = __builtin_amdgcn_inverse_ballot_w64(Y);
+  const bool zo = __builtin_amdgcn_inverse_ballot_w64(Z);
   // constant entry indices only, so the masks and bases stay in scalar registers
-  // (values are copied out before selecting: a select between array elements
-  // would select their addresses and keep the record in scratch memory)
   constexpr int e00 = reach_entry(ox, 0, 0);
-  uint64_t m = L.fm[e00];
-  int32_t base = L.base[e00];
+  uint64_t m00 = 0;
+  uint32_t slot = shifted_slot<d00>(L.fm[e00], L.base[e00], m00);
+  uint64_t fluid = m00 & ~(Y | Z);
   if constexpr (oy != 0 && oz != 0) {
     static_assert(ox == 0, "D3Q19 links have at most two nonzero components");
     constexpr int e10 = reach_entry(0, oy, 0), e01 = reach_entry(0, 0, oz), e11 = reach_entry(0, oy, oz);
-    const uint64_t m10 = L.fm[e10], m01 = L.fm[e01], m11 = L.fm[e11];
-    const int32_t b10 = L.base[e10], b01 = L.base[e01], b11 = L.base[e11];
-    m = yo ? (zo ? m11 : m10) : (zo ? m01 : m);
-    base = yo ? (zo ? b11 : b10) : (zo ? b01 : base);
+    uint64_t m10 = 0, m01 = 0, m11 = 0;
+    const uint32_t s10 = shifted_slot<d00 - 64 * oy>(L.fm[e10], L.base[e10], m10);
+    const uint32_t s01 = shifted_slot<d00 - 8 * oz>(L.fm[e01], L.base[e01], m01);
+    const uint32_t s11 = shifted_slot<d00 - 64 * oy - 8 * oz>(L.fm[e11], L.base[e11], m11);
+    slot = yo ? (zo ? s11 : s10) : (zo ? s01 : slot);
+    fluid |= (m10 & (Y & ~Z)) | (m01 & (Z & ~Y)) | (m11 & (Y & Z));
   } else if constexpr (oy != 0) {
     constexpr int e10 = reach_entry(ox, oy, 0);
-    const uint64_t m10 = L.fm[e10];
-    const int32_t b10 = L.base[e10];
-    m = yo ? m10 : m;
-    base = yo ? b10 : base;
+    uint64_t m10 = 0;
+    const uint32_t s10 = shifted_slot<d00 - 64 * oy>(L.fm[e10], L.base[e10], m10);
+    slot = yo ? s10 : slot;
+    fluid |= m10 & Y;
   } else if constexpr (oz != 0) {
     constexpr int e01 = reach_entry(ox, 0, oz);
-    const uint64_t m01 = L.fm[e01];
-    const int32_t b01 = L.base[e01];
-    m = zo ? m01 : m;
-    base = zo ? b01 : base;
+    uint64_t m01 = 0;
+    const uint32_t s01 = shifted_slot<d00 - 8 * oz>(L.fm[e01], L.base[e01], m01);
+    slot = zo ? s01 : slot;
+    fluid |= m01 & Z;
   }
-  const uint32_t pos = ((uint32_t)(yy & 7) << 3) | (uint32_t)(zz & 7);
-  const uint64_t bit = 1ull << pos;
-  const uint32_t slot = (uint32_t)base + (uint32_t)__builtin_popcountll(m & (bit - 1ull));
-  return (m & bit) ? slot : L.slot;
+  return __builtin_amdgcn_inverse_ballot_w64(fluid) ? slot : L.slot;
 }
 
 }  // namespace hlb

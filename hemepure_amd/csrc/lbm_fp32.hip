@@ -121,7 +121,7 @@ __device__ __forceinline__ Planes member_planes(const Member32& m, int step) {
 template <int J = 1>
 __device__ __forceinline__ void gather_up32(const float* in, int64_t S, const Lane& L, float* g) {
   if constexpr (J < Q) {
-    g[J] = ldp(plane(in, S, J), upstream_slot<J>(L));
+    g[J] = ldp(uplane(in, S, J), upstream_slot<J>(L));
     gather_up32<J + 1>(in, S, L, g);
   }
 }
@@ -133,7 +133,7 @@ __device__ __forceinline__ void boundary_fixup32(const Member32& m, const float*
                                                  uint32_t wm, uint32_t im, float* g) {
   float own[Q];
 #pragma unroll
-  for (int j = 1; j < Q; ++j) own[j] = ldp(plane(in, m.S, j), d);
+  for (int j = 1; j < Q; ++j) own[j] = ldp(uplane(in, m.S, j), d);
   const bool bfl = m.wall_bc == HLB_WALL_BFL;
   float q[Q];
   if (bfl && wm != 0) {
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1
   float g[Q];
   if (P.first) {
 #pragma unroll
-    for (int j = 0; j < Q; ++j) g[j] = ldp(plane(P.in, m.S, j), d);
+    for (int j = 0; j < Q; ++j) g[j] = ldp(uplane(P.in, m.S, j), d);
   } else {
     g[0] = ldp(P.in, d);
     gather_up32(P.in, m.S, L, g);
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1
   }
 
 #pragma unroll
-  for (int j = 0; j < Q; ++j) stp(plane(P.out, m.S, j), d, gpost[j]);
+  for (int j = 0; j < Q; ++j) stp(uplane(P.out, m.S, j), d, gpost[j]);
 
   const bool props = step == m.props_at;
   float w = FLT_MAX;
@@ -510,10 +510,10 @@ hipError_t launch_step_fp32(const Member32* members, const int32_t* wg_member, u
     case 4: HLB_V(512, false, W); break; \
     default: HLB_V(512, true, W); break; \
   }
-  // the FP32 kernel fits 5 waves per SIMD as compiled; levels 4, 5 ask for 6, 8
+  // the FP32 kernel fits 5 waves per SIMD as compiled; levels 4, 5 ask for 6, 7
   switch (fp32_variant_level(variant)) {
     case 4: HLB_SHAPES(6); break;
-    case 5: HLB_SHAPES(8); break;
+    case 5: HLB_SHAPES(7); break;
     default: HLB_SHAPES(0); break;
   }
 #undef HLB_SHAPES

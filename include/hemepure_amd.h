@@ -178,7 +178,7 @@ int hlb_step(hlb_handle h, int64_t nsteps, uint32_t flags);
 /* Launch variant of the steady-state step, 0..35 = shape + 6 * level: shape
  * 0..5 = workgroups of 128/256/512 slots without/with the XCD-contiguous
  * order, level 0..3 = at most (unlimited, 4, 3, 2) waves per SIMD, level
- * 4..5 = compiled for at least (5, 6) waves per SIMD in FP64 and (6, 8) in
+ * 4..5 = compiled for at least (5, 6) waves per SIMD in FP64 and (6, 7) in
  * the FP32-shifted form; default 1.
  * No counterpart in the reference.  hlb_autotune times every variant over
  * trial_steps real steps of the lattice (they advance it) and keeps the
