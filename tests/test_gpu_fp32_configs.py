@@ -1,14 +1,15 @@
 """FP32-shifted error against the FP64 path at the benchmark sizes (FP64 on
 the device is bit-identical to the oracle, so it stands in for it here):
-the Y-bifurcation (TRT, BFL, Ladd sinusoidal inlet) over 2000 steps and
-members of the stenosed-pipe ensemble over 5000 steps, each against its own
+the Y-bifurcation (TRT, BFL, Ladd sinusoidal inlet) over 2000 steps,
+members of the stenosed-pipe ensemble over 5000 steps and the sparse vessel
+tree (6.4M sites, LES, WK3 outlets) over 1000 steps, each against its own
 FP64 run.  Bar: max |x32 - x64| / max |x64| <= 1e-4 for velocity and WSS, and
 for rho - 1."""
 import numpy as np
 import pytest
 
 import hemepure_amd as H
-from fixtures import bifurcation_case, ensemble_cases
+from fixtures import bifurcation_case, ensemble_cases, vessel_tree_case
 
 pytestmark = pytest.mark.gpu
 
@@ -49,3 +50,9 @@ def test_ensemble_members_5000_steps():
         worst = {k: max(worst[k], e[k]) for k in worst}
     print(f"[fp32 vs fp64] ensemble worst: {worst}", flush=True)
     assert max(worst.values()) <= 1e-4
+
+
+def test_vessel_tree_full_size_1000_steps():
+    case = vessel_tree_case(threads=8)
+    e = errors(case, 1000, f"vessel tree {case.geom.n_sites} sites, 1000 steps")
+    assert max(e.values()) <= 1e-4
