@@ -91,6 +91,41 @@ def closed_box(nx: int, ny: int, nz: int, seed: int = 3) -> H.Geometry:
     return H.Geometry(tuple(blocks), coords, lt, li, ld, na, nrm, [])
 
 
+def sparse_blob(n: int = 40, fill: float = 0.45, spheres: int = 6, seed: int = 5, block_size: int = 6) -> H.Geometry:
+    """Random sparse fluid in an n^3 box: a few spheres joined with single
+    random voxels, so 8^3 bricks hold slabs of every fill (1..64 sites) and
+    a slab's neighbours in each direction are full, partial or missing.
+    Every link to a non-fluid position is a wall link with a seeded q (no
+    iolets), so the geometry is consistent; block_size 6 puts brick edges
+    across the file's blocks."""
+    rng = np.random.default_rng(seed)
+    bs = block_size
+    blocks = [math.ceil((n + 2) / bs)] * 3
+    grid = np.zeros((n + 2, n + 2, n + 2), dtype=bool)
+    inner = rng.uniform(size=(n, n, n)) < fill * 0.5
+    grid[1:n + 1, 1:n + 1, 1:n + 1] = inner
+    xs, ys, zs = np.meshgrid(np.arange(n + 2), np.arange(n + 2), np.arange(n + 2), indexing="ij")
+    for _ in range(spheres):
+        c = rng.uniform(4, n - 3, size=3)
+        r = rng.uniform(3.0, n / 4)
+        grid |= ((xs - c[0]) ** 2 + (ys - c[1]) ** 2 + (zs - c[2]) ** 2) <= r * r
+    grid[0, :, :] = grid[-1, :, :] = grid[:, 0, :] = grid[:, -1, :] = grid[:, :, 0] = grid[:, :, -1] = False
+    coords = np.argwhere(grid)
+    m = coords.shape[0]
+    lt = np.zeros((m, 18), dtype=np.uint8)
+    ld = np.full((m, 18), -1.0, dtype=np.float32)
+    for d in range(1, 19):
+        nb = coords + np.array([CX[d], CY[d], CZ[d]])
+        solid = ~grid[nb[:, 0], nb[:, 1], nb[:, 2]]
+        lt[solid, d - 1] = H.LINK_WALL
+        ld[solid, d - 1] = rng.uniform(0.05, 0.95, size=int(solid.sum())).astype(np.float32)
+    li = np.full((m, 18), -1, dtype=np.int32)
+    na = (lt == H.LINK_WALL).any(axis=1).astype(np.uint8)
+    nrm = np.zeros((m, 3), dtype=np.float32)
+    nrm[na == 1] = (0.48, 0.6, 0.64)
+    return H.Geometry(tuple(blocks), coords, lt, li, ld, na, nrm, [], block_size=bs)
+
+
 def anisotropic_f(n: int) -> np.ndarray:
     """LbTestsHelper::InitialiseAnisotropicTestData (src/unittests/lbtests/LbTestsHelper.h:193-199)."""
     d = np.arange(19)

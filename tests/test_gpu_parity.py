@@ -11,7 +11,7 @@ import pytest
 
 import hemepure_amd as H
 from fixtures import (Case, IoletSpec, anisotropic_f, bifurcation_case, closed_box, cylinder_case, four_cube,
-                      sine_velocity_table, windkessel_cylinder_case)
+                      sine_velocity_table, sparse_blob, windkessel_cylinder_case)
 
 pytestmark = pytest.mark.gpu
 
@@ -85,6 +85,25 @@ def test_four_cube_corners_anisotropic(kernel, wall_bc):
     for steps in (1, 2, 7):
         lat, ref = run_both(case, steps, f0=f0, t0=3)
         compare_all(lat, ref, f"four_cube k{kernel} w{wall_bc} s{steps}")
+
+
+@pytest.mark.parametrize("kernel,wall_bc", [(H.KERNEL_LBGK, H.WALL_SBB), (H.KERNEL_TRT, H.WALL_BFL),
+                                             (H.KERNEL_LES, H.WALL_BFL)])
+def test_sparse_blob_ragged_slabs(kernel, wall_bc):
+    """Random sparse fluid: slabs of every fill (1..64 sites, including the
+    32-site alignment boundary), neighbour slabs full, partial or missing on
+    every face, brick edges across the file's 6^3 blocks; anisotropic start.
+    FP64 bitwise against the oracle; FP32-shifted within 1e-4 of it."""
+    g = sparse_blob()
+    case = Case(g, kernel=kernel, wall_bc=wall_bc, tau=0.7)
+    f0 = 0.05 + 0.01 * anisotropic_f(g.n_sites)
+    lat, ref = run_both(case, 25, f0=f0, t0=1)
+    compare_all(lat, ref, f"sparse blob k{kernel} w{wall_bc}")
+    lat32 = case.product(precision=H.FP32_SHIFTED)
+    lat32.upload_f(f0, 1)
+    lat32.step(25, properties=True)
+    f32, f64 = lat32.download_f(), ref.f()
+    assert np.abs(f32 - f64).max() <= 1e-4 * np.abs(f64).max()
 
 
 def test_ladd_parabolic_with_warmup():

@@ -10,7 +10,7 @@ import pytest
 
 import hemepure_amd as H
 import oracle as O
-from fixtures import CX, CY, CZ, INV, Case, IoletSpec, closed_box, four_cube
+from fixtures import CX, CY, CZ, INV, Case, IoletSpec, closed_box, four_cube, sparse_blob
 
 
 def _geoms():
@@ -20,6 +20,7 @@ def _geoms():
     yield "cylinder", H.Geometry.cylinder(10.0, 100, 24)
     yield "stenosis", H.Geometry.stenosed_pipe(7.0, 60, 0.5, 7.0)
     yield "ybif_small", H.Geometry.ybifurcation(8.0, 40.0, 6.5, 32.0, 30.0, 0.03, 2026)
+    yield "sparse_blob", sparse_blob()
 
 
 def _handle(g):
