@@ -8,14 +8,16 @@ parabolic velocity inlet driven by a sinusoidal peak-speed table (peak 0.05,
 period 4000) and two Nash pressure outlets at rho = 1, FP64 storage.
 
 The same workload is then timed in the FP32-shifted storage form (float
-g = f - w rho0, 19 x 4 B read + written = 152 B per site update), reported
-under "forms".
+g = f - w rho0), reported under "forms".
 
 Prints one JSON line (rank 0, N = 1).  MFLUPS = sites x steps / seconds / 1e6.
 The roofline object prices the step kernel against MI355X's 8 TB/s HBM with
-the algorithmic byte count per site update (FP64: 19 reads + 19 writes of 8 B
-= 304 B; the slab layout finds neighbours arithmetically, so no index table
-is read).  cpu_baseline times the CPU oracle (the
+the algorithmic byte count of a pull step per site update: 19 reads + 19
+writes of the distributions plus an 18 x 4 B neighbour index, 376 B in FP64
+and 224 B in FP32.  This layout finds neighbours arithmetically and reads no
+index, so it moves 304 B / 152 B of distributions per update
+("layout_bytes_per_site_update"); "traffic" is the measured bytes per
+launch.  cpu_baseline times the CPU oracle (the
 reference's push/AoS FP64 algorithm, OpenMP over site ranges) on a bounded
 sample of the same workload.
 """
@@ -36,7 +38,11 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "tests"))
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_SITE = {"f64": 19 * 8 * 2, "f32": 19 * 4 * 2}
+# algorithmic bytes of a pull step per site update: distributions read and
+# written plus an 18 x 4 B neighbour index
+BYTES_PER_SITE = {"f64": 19 * 8 * 2 + 18 * 4, "f32": 19 * 4 * 2 + 18 * 4}
+# what this layout itself moves: the distributions only (no index table)
+LAYOUT_BYTES_PER_SITE = {"f64": 19 * 8 * 2, "f32": 19 * 4 * 2}
 
 
 def log(msg: str) -> None:
@@ -464,7 +470,8 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": round(traffic["bytes_per_launch"]) if traffic else _committed_traffic(args.workload, n),
-                     "bytes_per_site_update": BYTES_PER_SITE["f64"], "kernel": "step_fp64"},
+                     "bytes_per_site_update": BYTES_PER_SITE["f64"],
+                     "layout_bytes_per_site_update": LAYOUT_BYTES_PER_SITE["f64"], "kernel": "step_fp64"},
     }
     out["forms"] = {
         "fp64": {"mflups": round(mflups, 2), "ms_per_step": ms_total / args.steps, "launch_variant": variant64,
