@@ -136,16 +136,17 @@ __device__ __forceinline__ uint32_t shifted_slot(uint64_t m, int32_t base, uint6
 constexpr uint64_t kLanesY0 = 0x00000000000000ffull, kLanesY7 = 0xff00000000000000ull;
 constexpr uint64_t kLanesZ0 = 0x0101010101010101ull, kLanesZ7 = 0x8080808080808080ull;
 
-// Slot of the upstream site x - c_J of a lane (pull direction J), or the
-// lane's own slot when that position is not fluid (the link is a wall or
-// iolet link; its boundary delegate replaces the value).  The position is
+// Slot of the upstream site x - c_J of a lane (pull direction J) where that
+// position is fluid; fluid_lanes receives the lane mask of the lanes for
+// which it is (for the others the link is a wall or iolet link and the slot
+// is meaningless; upstream_slot substitutes the lane's own).  The position is
 // lane + 8 oy + oz in the slab the record names for the x step, or -- for
 // lanes whose y / z step leaves the brick -- 64 or 8 positions back in the
 // neighbouring brick's slab: at most four candidates, each one mbcnt pair,
 // picked with constant lane masks; the fluid test combines the candidates'
 // shifted masks with scalar logic into one lane mask.
 template <int J>
-__device__ __forceinline__ uint32_t upstream_slot(const Lane& L) {
+__device__ __forceinline__ uint32_t upstream_pos(const Lane& L, uint64_t& fluid_lanes) {
   constexpr int ox = -CX[J], oy = -CY[J], oz = -CZ[J];
   constexpr int d00 = 8 * oy + oz;
   // lanes whose step leaves the brick in y / in z
@@ -180,6 +181,14 @@ __device__ __forceinline__ uint32_t upstream_slot(const Lane& L) {
     slot = zo ? s01 : slot;
     fluid |= m01 & Z;
   }
+  fluid_lanes = fluid;
+  return slot;
+}
+
+template <int J>
+__device__ __forceinline__ uint32_t upstream_slot(const Lane& L) {
+  uint64_t fluid = 0;
+  const uint32_t slot = upstream_pos<J>(L, fluid);
   return __builtin_amdgcn_inverse_ballot_w64(fluid) ? slot : L.slot;
 }
 

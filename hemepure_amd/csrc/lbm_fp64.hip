@@ -109,6 +109,73 @@ __device__ __forceinline__ void boundary_fixup(const StepArgs& a, uint32_t d, ui
   }
 }
 
+// The steady gather when the 19 planes of a lattice fit 32-bit element
+// offsets from plane 0 (19 S < 2^29): a direction whose upstream position is
+// not fluid (a wall or iolet link ib = INV[J]) reads the lane's own value in
+// plane ib -- the bounce-back value SBB keeps, BFL keeps or interpolates
+// from, an iolet or GZS link replaces -- so the boundary delegates load no
+// own row.  One load per direction either way: the plane is picked per lane
+// through the offset.
+template <int J = 1>
+__device__ __forceinline__ void gather_up_bb(const StepArgs& a, const Lane& L, double* f) {
+  if constexpr (J < Q) {
+    uint64_t fluid = 0;
+    const uint32_t up = upstream_pos<J>(L, fluid);
+    const uint32_t pj = (uint32_t)(J * a.S), pi = (uint32_t)(INV[J] * a.S);
+    f[J] = ldp(a.ain[0], __builtin_amdgcn_inverse_ballot_w64(fluid) ? pj + up : pi + L.slot);
+    gather_up_bb<J + 1>(a, L, f);
+  }
+}
+
+// boundary_fixup for gather_up_bb: f[j] already holds own[ib] for every wall
+// or iolet link ib; own[j] is read only where BFL interpolates.  Same
+// operations, in the same order, as boundary_fixup.
+__device__ __forceinline__ void boundary_fixup_bb(const StepArgs& a, uint32_t d, uint32_t b, uint32_t wm,
+                                                  uint32_t im, double* f) {
+  const bool bfl = a.wall_bc == HLB_WALL_BFL;
+  float q[Q];
+  if (bfl && wm != 0) {
+#pragma unroll
+    for (int k = 1; k < Q; ++k) q[k] = ldp(a.bq + (int64_t)(k - 1) * a.SB, b);
+  }
+  if (im != 0) {
+    const uint32_t i = (uint32_t)ldp(a.bii, b);
+#pragma unroll
+    for (int j = 1; j < Q; ++j)
+      if (im & (1u << (INV[j] - 1))) f[j] = ldp(plane(a.G, a.SI, j), i);
+  }
+  uint32_t gzm = 0;
+  int32_t gzb = 0;
+  if (a.gz.bgz != nullptr && wm != 0) {
+    gzm = ldp(a.gz.bgz, b);
+    gzb = ldp(a.gz.bgzbase, b);
+  }
+  if (wm != 0) {
+#pragma unroll
+    for (int j = 1; j < Q; ++j) {
+      const int ib = INV[j];
+      const uint32_t bit = 1u << (ib - 1), bitJ = 1u << (j - 1);
+      if ((wm & bit) && !(im & bit)) {
+        if (gzm & bit) {
+          f[j] = ldp(a.GW, (uint32_t)(gzb + __builtin_popcount(gzm & (bit - 1u))));
+          continue;
+        }
+        const double qd = bfl ? (double)q[ib] : 0.0;
+        if (bfl && !(wm & bitJ) && !(qd < 0.5)) f[j] = (f[j] + (2.0 * qd - 1) * ldp(a.ain[j], d)) / (2.0 * qd);
+      }
+    }
+    if (bfl) {
+#pragma unroll
+      for (int k = 1; k < Q; ++k) {
+        const int j = INV[k];
+        const double qk = (double)q[k];
+        if ((wm & (1u << (k - 1))) && !(wm & (1u << (j - 1))) && qk < 0.5)
+          f[j] = 2.0 * qk * f[j] + (1.0 - 2.0 * qk) * f[k];
+      }
+    }
+  }
+}
+
 // Iolet delegates of one site: the value each iolet link k hands to the next
 // step in direction INV[k].  Nash: f_eq of the ghost state
 // (NashZerothOrderPressureDelegate.h:36-76).  Ladd: bounce-back minus the
@@ -228,7 +295,9 @@ __device__ __forceinline__ uint32_t slab_record() {
 }
 
 // B threads (B / 64 slabs) per workgroup; REMAP: XCD-contiguous tile order.
-template <int KERNEL, bool FIRST, bool PROPS, int B = kBlock, bool REMAP = true, int W = 0>
+// NARROW: the lattice's planes fit 32-bit offsets from plane 0 (19 S < 2^29),
+// so the steady step uses gather_up_bb / boundary_fixup_bb.
+template <int KERNEL, bool FIRST, bool PROPS, int B = kBlock, bool REMAP = true, int W = 0, bool NARROW = false>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1))) void step_fp64(StepArgs a) {
   Lane L;
   const uint32_t tile = REMAP ? xcd_tile() : blockIdx.x;
@@ -245,6 +314,10 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1
     // Initial condition or checkpoint: f is the pre-collision state itself.
 #pragma unroll
     for (int j = 0; j < Q; ++j) f[j] = ldp(a.ain[j], d);
+  } else if constexpr (NARROW) {
+    f[0] = ldp(a.ain[0], L.slot);
+    gather_up_bb(a, L, f);
+    if (isB) boundary_fixup_bb(a, d, L.b, wm, im, f);
   } else {
     gather_site(a, L, f);
     if (isB) boundary_fixup(a, d, L.b, wm, im, f);
@@ -653,13 +726,13 @@ static unsigned slab_grid(int64_t nslab) {
 
 // Steady-state steps in launch shape `variant` (as the FP32 kernel's); the
 // first step after an initial state and property steps use 256-thread tiles.
-template <int KN>
+template <int KN, bool NARROW>
 void launch_variant_fp64(const StepArgs& a, int variant, hipStream_t s) {
   const int per = fp32_variant_slabs(variant);
   const int64_t tiles = (a.bt.nslab + per - 1) / per;
   const dim3 grid((unsigned)((tiles + kXcds - 1) / kXcds * kXcds));
   const unsigned lds = fp32_variant_lds(variant);
-#define HLB_V(B, R, W) hipLaunchKernelGGL((step_fp64<KN, false, false, B, R, W>), grid, dim3(B), lds, s, a)
+#define HLB_V(B, R, W) hipLaunchKernelGGL((step_fp64<KN, false, false, B, R, W, NARROW>), grid, dim3(B), lds, s, a)
 #define HLB_SHAPES(W)                    \
   switch (variant % kFp32Shapes) {       \
     case 0: HLB_V(128, false, W); break; \
@@ -682,11 +755,18 @@ void launch_variant_fp64(const StepArgs& a, int variant, hipStream_t s) {
 hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool props, int variant, hipStream_t s) {
   if (a.bt.nslab <= 0) return hipSuccess;
   if (!first && !props) {
+    // 32-bit element offsets from plane 0 cover all 19 planes (the byte
+    // offset i * 8 stays below 2^32), and the planes are one allocation
+    const bool narrow = 19 * a.S < (int64_t(1) << 29) && a.ain[18] == a.ain[0] + 18 * a.S;
+#define HLB_K(KN)                                          \
+  if (narrow) launch_variant_fp64<KN, true>(a, variant, s); \
+  else launch_variant_fp64<KN, false>(a, variant, s);
     switch (kernel) {
-      case HLB_KERNEL_LBGK: launch_variant_fp64<HLB_KERNEL_LBGK>(a, variant, s); break;
-      case HLB_KERNEL_TRT: launch_variant_fp64<HLB_KERNEL_TRT>(a, variant, s); break;
-      default: launch_variant_fp64<HLB_KERNEL_LES>(a, variant, s); break;
+      case HLB_KERNEL_LBGK: HLB_K(HLB_KERNEL_LBGK); break;
+      case HLB_KERNEL_TRT: HLB_K(HLB_KERNEL_TRT); break;
+      default: HLB_K(HLB_KERNEL_LES); break;
     }
+#undef HLB_K
     return hipGetLastError();
   }
   const dim3 grid(slab_grid(a.bt.nslab)), block(kBlock);
