@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstdlib>
 
 #include "../../include/hemepure_amd.h"
 #include "d3q19.h"
@@ -757,7 +758,11 @@ hipError_t launch_step_fp64(const StepArgs& a, int kernel, bool first, bool prop
   if (!first && !props) {
     // 32-bit element offsets from plane 0 cover all 19 planes (the byte
     // offset i * 8 stays below 2^32), and the planes are one allocation
-    const bool narrow = 19 * a.S < (int64_t(1) << 29) && a.ain[18] == a.ain[0] + 18 * a.S;
+    // HLB_FP64_GENERAL_GATHER=1 selects the general gather on any lattice (the
+    // path lattices above 28M slots take), so tests cover it on small ones.
+    const char* force = std::getenv("HLB_FP64_GENERAL_GATHER");
+    const bool narrow = !(force && force[0] == '1') && 19 * a.S < (int64_t(1) << 29) &&
+                        a.ain[18] == a.ain[0] + 18 * a.S;
 #define HLB_K(KN)                                          \
   if (narrow) launch_variant_fp64<KN, true>(a, variant, s); \
   else launch_variant_fp64<KN, false>(a, variant, s);

@@ -56,3 +56,4 @@ def test_vessel_tree_full_size_1000_steps():
     case = vessel_tree_case(threads=8)
     e = errors(case, 1000, f"vessel tree {case.geom.n_sites} sites, 1000 steps")
     assert max(e.values()) <= 1e-4
+

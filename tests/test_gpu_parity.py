@@ -106,6 +106,25 @@ def test_sparse_blob_ragged_slabs(kernel, wall_bc):
     assert np.abs(f32 - f64).max() <= 1e-4 * np.abs(f64).max()
 
 
+def test_general_fp64_gather_path(monkeypatch):
+    """FP64 lattices above 28M slots (19 S >= 2^29) gather with per-plane
+    offsets and load the own row for wall links; HLB_FP64_GENERAL_GATHER=1
+    selects that path here.  Bitwise against the oracle on FourCube (every
+    class, q on both sides of 1/2) and the small bifurcation (TRT, BFL,
+    Ladd, Nash)."""
+    monkeypatch.setenv("HLB_FP64_GENERAL_GATHER", "1")
+    g = four_cube(8, seed=3)
+    case = Case(g, kernel=H.KERNEL_TRT, wall_bc=H.WALL_BFL, tau=0.7,
+                inlets=[IoletSpec(H.COSINE, (3.5, 3.5, 0.5), (0, 0, 1), density_mean=1.01, density_amp=0.002,
+                                  period=7.0)],
+                outlets=[IoletSpec(H.COSINE, (3.5, 3.5, 6.5), (0, 0, -1), density_mean=0.99)])
+    f0 = 0.05 + 0.01 * anisotropic_f(g.n_sites)
+    lat, ref = run_both(case, 7, f0=f0, t0=3)
+    compare_all(lat, ref, "four_cube general gather")
+    lat, ref = run_both(bifurcation_case(rp=8.0, lp=40.0, rd=6.5, ld=32.0, steps=400, peak=0.03, period=200.0), 300)
+    compare_all(lat, ref, "small bifurcation general gather")
+
+
 def test_ladd_parabolic_with_warmup():
     g = H.Geometry.cylinder(8.0, 48, 24)
     i0, o0 = g.inlets()[0], g.outlets()[0]
