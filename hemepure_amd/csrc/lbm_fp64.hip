@@ -135,9 +135,13 @@ __device__ __forceinline__ void boundary_fixup_bb(const StepArgs& a, uint32_t d,
                                                   uint32_t im, double* f) {
   const bool bfl = a.wall_bc == HLB_WALL_BFL;
   float q[Q];
-  if (bfl && wm != 0) {
 #pragma unroll
-    for (int k = 1; k < Q; ++k) q[k] = ldp(a.bq + (int64_t)(k - 1) * a.SB, b);
+  for (int k = 1; k < Q; ++k) q[k] = 0.0f;
+  if (bfl) {
+    // distances of the lane's wall links only (all loads first, then their uses)
+#pragma unroll
+    for (int k = 1; k < Q; ++k)
+      if (wm & (1u << (k - 1))) q[k] = ldp(a.bq + (int64_t)(k - 1) * a.SB, b);
   }
   if (im != 0) {
     const uint32_t i = (uint32_t)ldp(a.bii, b);
