@@ -483,7 +483,7 @@ hipError_t launch_cycle_delta(const double* u, float* prev, int64_t N, int64_t S
 
 int fp32_variant_block(int variant) {
   const int shape = variant % kFp32Shapes;
-  return shape / 2 == 0 ? 128 : (shape / 2 == 1 ? 256 : 512);
+  return shape / 2 == 0 ? 128 : (shape / 2 == 1 ? 256 : 64);
 }
 int fp32_variant_slabs(int variant) { return fp32_variant_block(variant) / kSlab; }
 bool fp32_variant_remap(int variant) { return ((variant % kFp32Shapes) & 1) != 0; }
@@ -507,8 +507,8 @@ hipError_t launch_step_fp32(const Member32* members, const int32_t* wg_member, u
     case 1: HLB_V(128, true, W); break;  \
     case 2: HLB_V(256, false, W); break; \
     case 3: HLB_V(256, true, W); break;  \
-    case 4: HLB_V(512, false, W); break; \
-    default: HLB_V(512, true, W); break; \
+    case 4: HLB_V(64, false, W); break; \
+    default: HLB_V(64, true, W); break; \
   }
   // the FP32 kernel fits 5 waves per SIMD as compiled; levels 4, 5 ask for 6, 7
   switch (fp32_variant_level(variant)) {

@@ -744,8 +744,8 @@ void launch_variant_fp64(const StepArgs& a, int variant, hipStream_t s) {
     case 1: HLB_V(128, true, W); break;  \
     case 2: HLB_V(256, false, W); break; \
     case 3: HLB_V(256, true, W); break;  \
-    case 4: HLB_V(512, false, W); break; \
-    default: HLB_V(512, true, W); break; \
+    case 4: HLB_V(64, false, W); break; \
+    default: HLB_V(64, true, W); break; \
   }
   // the FP64 kernel fits 4 waves per SIMD as compiled; levels 4, 5 ask for 5, 6
   switch (fp32_variant_level(variant)) {

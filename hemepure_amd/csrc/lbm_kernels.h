@@ -77,7 +77,7 @@ struct BrickTables {
 constexpr int kMaxIolets = 256;  // iolets per lattice
 
 // Launch shapes of the step kernels: variant = shape + 6 * level.  Shape
-// 0..5: workgroup tile of 2, 4 or 8 slabs (128, 256, 512 threads), each
+// 0..5: workgroup tile of 2, 4 or 1 slabs (128, 256, 64 threads), each
 // without and with the XCD-contiguous tile order.  Level 0..3 caps the
 // waves resident per SIMD (none, 4, 3, 2) by giving each workgroup dynamic
 // LDS it does not use: fewer waves in flight keep more of each wave's

@@ -176,7 +176,7 @@ int hlb_upload_f(hlb_handle h, const double* f, int64_t t0);
  * src/lb/streamers/BaseStreamer.h:88-120).  Asynchronous. */
 int hlb_step(hlb_handle h, int64_t nsteps, uint32_t flags);
 /* Launch variant of the steady-state step, 0..35 = shape + 6 * level: shape
- * 0..5 = workgroups of 128/256/512 slots without/with the XCD-contiguous
+ * 0..5 = workgroups of 128/256/64 slots without/with the XCD-contiguous
  * order, level 0..3 = at most (unlimited, 4, 3, 2) waves per SIMD, level
  * 4..5 = compiled for at least (5, 6) waves per SIMD in FP64 and (6, 7) in
  * the FP32-shifted form; default 1.
